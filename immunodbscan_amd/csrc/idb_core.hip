@@ -1,0 +1,1103 @@
+// idb_core.hip — host driver and O(N) kernels of libidb_hip.so (gfx950).
+//
+// Pipeline of one call (stage names follow the reference's timers,
+// cuda/PSDBSCAN.hcu:1301-1354):
+//  Identify Duplicates      row scan (64-bit content hash, letter set, max
+//                           length) -> radix sort of (hash, row) -> each row's
+//                           representative = first equal row of its hash run.
+//                           Replaces the O(N^2) singlet_find_duplicates_kernel
+//                           (:525-573) with O(N log N) work; equality is
+//                           checked on the full content, so collisions only
+//                           cost time.
+//  Determine copies         class sizes, uniques (ascending), count init (:1074-1111)
+//  Identify core points     bucket entries (unique row x V gene x J gene),
+//                           radix sort by (repertoire, bucket, length), tile
+//                           list, count pass (idb_pairs.hip), core flags (:1097-1184)
+//  Grow clusters            fuse pass over the same tiles (:1189-1260)
+//  Reduction of cluster information tree structure
+//                           roots of cores, border roots (:1007-1017)
+//  Re-number clusters and identify noise
+//                           exclusive scan of root flags -> labels (:1263-1294)
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../../include/idb.h"
+#include "idb_internal.h"
+
+// ---------------------------------------------------------------------------
+// Context and device arena
+enum Slot {
+    S_SEQ, S_OFF, S_LEN, S_V, S_J, S_RID, S_RSTART,
+    S_KEY, S_IDX, S_KEY2, S_IDX2, S_HEAD, S_REP, S_COPIES, S_CNT, S_CORE, S_PAR, S_BMIN,
+    S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
+    S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
+    S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
+    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART,
+    S_COUNT
+};
+
+struct idb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    void *ptr[S_COUNT] = {};
+    size_t cap[S_COUNT] = {};
+    uint64_t *pinned = nullptr;   // small host staging area for counters
+    hipEvent_t ev[IDB_N_STAGES + 1] = {};
+    hipEvent_t kev[4] = {};       // around the count and the fuse kernels
+};
+
+namespace {
+
+struct Fail {
+    int code;
+};
+
+void check(idb_ctx *c, hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        c->err = std::string(what) + ": " + hipGetErrorString(e);
+        throw Fail{IDB_ERR_HIP};
+    }
+}
+
+void fail(idb_ctx *c, int code, const std::string &msg)
+{
+    c->err = msg;
+    throw Fail{code};
+}
+
+template <typename T>
+T *buf(idb_ctx *c, Slot s, size_t count)
+{
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 256;
+    if (c->cap[s] < bytes) {
+        if (c->ptr[s]) (void)hipFree(c->ptr[s]);
+        c->ptr[s] = nullptr;
+        c->cap[s] = 0;
+        size_t want = bytes + bytes / 4;
+        void *p = nullptr;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                fail(c, IDB_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+            }
+            want = bytes;
+        }
+        c->ptr[s] = p;
+        c->cap[s] = want;
+    }
+    return (T *)c->ptr[s];
+}
+
+void *tmp(idb_ctx *c, size_t bytes) { return buf<uint8_t>(c, S_TMP, bytes); }
+
+// ---------------------------------------------------------------------------
+// R5 threshold table.  The reference computes truncf(max(float lx, float ly)
+// * eps) on the device and converts to int with the device's saturating rule
+// (NaN -> 0); it depends on the pair only through max(lx, ly).
+int32_t cvt_rzi_sat(float f)
+{
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+
+int32_t threshold_of(uint32_t L, const idb_params &p)
+{
+    if (p.abs_k >= 0) return p.abs_k;
+    volatile float prod = (float)L * p.eps;
+    return cvt_rzi_sat(std::trunc((float)prod));
+}
+
+int16_t clamp16(int64_t v)
+{
+    if (v < -1) return -1;
+    if (v > 32767) return 32767;
+    return (int16_t)v;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+__device__ __forceinline__ uint64_t mix64(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h *= 0xBF58476D1CE4E5B9ull;
+    return h ^ (h >> 31);
+}
+
+__device__ __forceinline__ uint64_t fin64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_fill_rid(const uint32_t *rstart, uint32_t n_reps, uint32_t n, uint32_t *rid)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        // last repertoire whose first row is <= i (empty repertoires are skipped)
+        uint32_t lo = 0, hi = n_reps;
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (rstart[mid] <= i) lo = mid; else hi = mid;
+        }
+        rid[i] = lo;
+    }
+}
+
+// Row scan: content hash (R3 key), letter set, longest row.
+__global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
+                          const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t n, uint64_t hmask,
+                          uint64_t *key, uint32_t *idx, uint32_t *small)
+{
+    __shared__ uint32_t sym[8];
+    __shared__ uint32_t smax;
+    if (threadIdx.x < 8) sym[threadIdx.x] = 0;
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t L = len[i];
+        const uint8_t *s = seq + off[i];
+        uint64_t h = mix64(0x1234567ull, ((uint64_t)L << 32) | (rid ? rid[i] : 0u));
+        uint64_t acc = 0;
+        for (uint32_t k = 0; k < L; ++k) {
+            const uint32_t b = s[k];
+            atomicOr(&sym[b >> 5], 1u << (b & 31));
+            acc = (acc << 8) | b;
+            if ((k & 7) == 7) {
+                h = mix64(h, acc);
+                acc = 0;
+            }
+        }
+        h = mix64(h, acc);
+        for (uint32_t w = 0; w < vw; ++w) h = mix64(h, (uint32_t)vb[(size_t)i * vw + w]);
+        for (uint32_t w = 0; w < jw; ++w) h = mix64(h, 0x100000000ull | (uint32_t)jb[(size_t)i * jw + w]);
+        key[i] = fin64(h) & hmask;
+        idx[i] = i;
+        atomicMax(&smax, L);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && sym[threadIdx.x]) atomicOr(&small[threadIdx.x], sym[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(&small[8], smax);
+}
+
+__global__ void k_seghead(const uint64_t *skey, uint32_t n, uint32_t *head)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        head[p] = (p == 0 || skey[p] != skey[p - 1]) ? p : 0u;
+}
+
+__device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
+                           const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t a, uint32_t b)
+{
+    // R3 (PSDBSCAN.hcu:542-565): equal length, bitsets and bytes; batched
+    // repertoires never share duplicates.
+    if (len[a] != len[b]) return false;
+    if (rid && rid[a] != rid[b]) return false;
+    for (uint32_t w = 0; w < vw; ++w)
+        if (vb[(size_t)a * vw + w] != vb[(size_t)b * vw + w]) return false;
+    for (uint32_t w = 0; w < jw; ++w)
+        if (jb[(size_t)a * jw + w] != jb[(size_t)b * jw + w]) return false;
+    const uint8_t *sa = seq + off[a], *sb = seq + off[b];
+    for (uint32_t k = 0; k < len[a]; ++k)
+        if (sa[k] != sb[k]) return false;
+    return true;
+}
+
+// R4: representative = minimum index among equal rows.  Within a hash run the
+// stable sort keeps rows ascending, so the first equal row is the minimum.
+__global__ void k_dedup(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
+                        const int32_t *jb, uint32_t jw, const uint32_t *rid, const uint32_t *sidx, const uint32_t *head,
+                        uint32_t n, uint32_t *rep, uint32_t *copies)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t i = sidx[p];
+        uint32_t r = i;
+        for (uint32_t q = head[p]; q < p; ++q) {
+            const uint32_t j = sidx[q];
+            if (rows_equal(seq, off, len, vb, vw, jb, jw, rid, i, j)) {
+                r = j;
+                break;
+            }
+        }
+        rep[i] = r;
+        atomicAdd(copies + r, 1u);
+    }
+}
+
+__global__ void k_after_dedup(const uint32_t *rep, const uint32_t *copies, uint32_t n, uint8_t *flag, int32_t *cnt,
+                              uint32_t *par, uint32_t *bmin, uint32_t *rflag)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        flag[i] = rep[i] == i;
+        cnt[i] = (int32_t)copies[i];
+        par[i] = i;
+        bmin[i] = IDB_NONE;
+        rflag[i] = 0u;
+    }
+}
+
+__device__ __forceinline__ uint32_t popc_words(const int32_t *w, uint32_t n)
+{
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < n; ++k) c += __builtin_popcount((uint32_t)w[k]);
+    return c;
+}
+
+__global__ void k_membership(const uint32_t *uniq, const uint32_t *num, const int32_t *vb, uint32_t vw,
+                             const int32_t *jb, uint32_t jw, uint32_t single, uint64_t *memb)
+{
+    const uint32_t U = num[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= U; k += gridDim.x * blockDim.x) {
+        if (k == U) {
+            memb[k] = 0u;
+            continue;
+        }
+        const uint32_t u = uniq[k];
+        memb[k] = single ? 1u : (uint64_t)popc_words(vb + (size_t)u * vw, vw) * popc_words(jb + (size_t)u * jw, jw);
+    }
+}
+
+// One entry per (unique row, V gene, J gene); key = bucket << 16 | length.
+__global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const uint64_t *eoff, const int32_t *vb,
+                               uint32_t vw, const int32_t *jb, uint32_t jw, const uint32_t *len, const uint32_t *rid,
+                               uint32_t single, uint32_t gj, uint32_t gvj, uint64_t *ekey, uint32_t *erow)
+{
+    const uint32_t U = num[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < U; k += gridDim.x * blockDim.x) {
+        const uint32_t u = uniq[k];
+        const uint64_t r = rid ? rid[u] : 0u;
+        const uint64_t L = len[u];
+        uint64_t o = eoff[k];
+        if (single) {
+            ekey[o] = (r << 16) | L;
+            erow[o] = u;
+            continue;
+        }
+        for (uint32_t wv = 0; wv < vw; ++wv) {
+            uint32_t mv = (uint32_t)vb[(size_t)u * vw + wv];
+            while (mv) {
+                const uint32_t v = wv * 32 + __builtin_ctz(mv);
+                mv &= mv - 1;
+                for (uint32_t wj = 0; wj < jw; ++wj) {
+                    uint32_t mj = (uint32_t)jb[(size_t)u * jw + wj];
+                    while (mj) {
+                        const uint32_t j = wj * 32 + __builtin_ctz(mj);
+                        mj &= mj - 1;
+                        const uint64_t bucket = r * gvj + (uint64_t)v * gj + j;
+                        ekey[o] = (bucket << 16) | L;
+                        erow[o] = u;
+                        ++o;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ bool has_bit_below(const int32_t *w, uint32_t nw, uint32_t g)
+{
+    for (uint32_t k = 0; k < nw && k * 32 < g; ++k) {
+        uint32_t m = (uint32_t)w[k];
+        if (k * 32 + 32 > g) m &= (1u << (g - k * 32)) - 1u;
+        if (m) return true;
+    }
+    return false;
+}
+
+// Materialise entries in bucket order: row, length, copies, check flag, codes.
+__global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint32_t E, const uint8_t *seq,
+                               const uint64_t *off, const uint32_t *copies, const int32_t *vb, uint32_t vw,
+                               const int32_t *jb, uint32_t jw, uint32_t single, uint32_t gj, uint32_t gvj,
+                               const uint8_t *cmap, uint32_t S, uint32_t HW, uint32_t bits, uint16_t *elen,
+                               uint32_t *ecp, uint8_t *echk, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
+{
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        const uint64_t k = ekey[e];
+        const uint32_t u = erow[e];
+        const uint32_t L = (uint32_t)(k & 0xFFFFu);
+        const uint64_t bucket = k >> 16;
+        elen[e] = (uint16_t)L;
+        ecp[e] = copies[u];
+        bflag[e] = (e == 0) || ((ekey[e - 1] >> 16) != bucket);
+        if (single) {
+            echk[e] = 1;
+        } else {
+            const uint32_t bl = (uint32_t)(bucket % gvj);
+            const uint32_t v = bl / gj, j = bl % gj;
+            echk[e] = has_bit_below(vb + (size_t)u * vw, vw, v) || has_bit_below(jb + (size_t)u * jw, jw, j);
+        }
+        const uint8_t *s = seq + off[u];
+        if (ecodes) {
+            uint32_t *dst = ecodes + (size_t)e * (S / 4);
+            for (uint32_t w = 0; w < S / 4; ++w) {
+                uint32_t word = 0;
+                for (uint32_t b = 0; b < 4; ++b) {
+                    const uint32_t pos = w * 4 + b;
+                    if (pos < L) word |= (uint32_t)cmap[s[pos]] << (8 * b);
+                }
+                dst[w] = word;
+            }
+        }
+        if (epk) {
+            uint32_t *dst = epk + (size_t)e * HW;
+            const uint32_t per = 32 / bits;
+            for (uint32_t w = 0; w < HW; ++w) {
+                uint32_t word = 0;
+                for (uint32_t b = 0; b < per; ++b) {
+                    const uint32_t pos = w * per + b;
+                    if (pos < L) word |= (uint32_t)cmap[s[pos]] << (bits * b);
+                }
+                dst[w] = word;
+            }
+        }
+    }
+}
+
+// Query blocks per bucket.
+__global__ void k_bucket_blocks(const uint32_t *bstart, const uint32_t *nb_dev, uint32_t E, uint32_t *qbcnt)
+{
+    const uint32_t B = nb_dev[0];
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= B; b += gridDim.x * blockDim.x) {
+        if (b == B) {
+            qbcnt[b] = 0;
+            continue;
+        }
+        const uint32_t s = bstart[b], e = (b + 1 < B) ? bstart[b + 1] : E;
+        qbcnt[b] = (e - s + IDB_TILE - 1) / IDB_TILE;
+    }
+}
+
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t n, uint32_t v)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Index of the last entry in [s, e) with length <= Lmax, or s - 1.
+__device__ __forceinline__ int64_t last_len_le(const uint16_t *elen, uint32_t s, uint32_t e, uint32_t Lmax)
+{
+    uint32_t lo = s, hi = e;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (elen[mid] <= Lmax) lo = mid + 1; else hi = mid;
+    }
+    return (int64_t)lo - 1;
+}
+
+struct QBlock {
+    uint32_t b, I, s, e, nblk;
+};
+
+__device__ __forceinline__ QBlock qblock_of(uint32_t qb, const uint32_t *qboff, uint32_t B, const uint32_t *bstart,
+                                            uint32_t E)
+{
+    QBlock r;
+    r.b = upper_bound_u32(qboff, B, qb) - 1;
+    r.I = qb - qboff[r.b];
+    r.s = bstart[r.b];
+    r.e = (r.b + 1 < B) ? bstart[r.b + 1] : E;
+    r.nblk = (r.e - r.s + IDB_TILE - 1) / IDB_TILE;
+    return r;
+}
+
+__device__ __forceinline__ int64_t last_target_block(const QBlock &q, const uint16_t *elen, const uint16_t *hilen,
+                                                    uint32_t banded)
+{
+    if (!banded) return (int64_t)q.nblk - 1;
+    const uint32_t qend = min(q.s + (q.I + 1) * IDB_TILE, q.e);
+    const uint32_t lmax = elen[qend - 1];
+    const int64_t last = last_len_le(elen, q.s, q.e, hilen[lmax]);
+    if (last < (int64_t)q.s) return -1;
+    return (last - (int64_t)q.s) / IDB_TILE;
+}
+
+__global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
+                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint32_t *tcnt)
+{
+    const uint32_t B = nb_dev[0];
+    const uint32_t QB = qboff[B];
+    for (uint32_t qb = blockIdx.x * blockDim.x + threadIdx.x; qb <= QB; qb += gridDim.x * blockDim.x) {
+        if (qb == QB) {
+            tcnt[qb] = 0;
+            continue;
+        }
+        const QBlock q = qblock_of(qb, qboff, B, bstart, E);
+        const int64_t jhi = last_target_block(q, elen, hilen, banded);
+        tcnt[qb] = jhi >= (int64_t)q.I ? (uint32_t)(jhi - q.I + 1) : 0u;
+    }
+}
+
+__global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
+                             const uint16_t *elen, const uint16_t *hilen, uint32_t banded, const uint64_t *ekey,
+                             uint32_t gvj, const uint32_t *toff, idb_tile *tiles)
+{
+    const uint32_t B = nb_dev[0];
+    const uint32_t QB = qboff[B];
+    for (uint32_t qb = blockIdx.x * blockDim.x + threadIdx.x; qb < QB; qb += gridDim.x * blockDim.x) {
+        const QBlock q = qblock_of(qb, qboff, B, bstart, E);
+        const int64_t jhi = last_target_block(q, elen, hilen, banded);
+        const uint32_t q0 = q.s + q.I * IDB_TILE;
+        const uint32_t qn = min((uint32_t)IDB_TILE, q.e - q0);
+        // bucket id within its repertoire: the canonical test needs only (v, j)
+        const uint32_t bucket = (uint32_t)((ekey[q.s] >> 16) % gvj);
+        uint32_t o = toff[qb];
+        for (int64_t J = q.I; J <= jhi; ++J) {
+            const uint32_t t0 = q.s + (uint32_t)J * IDB_TILE;
+            const uint32_t tn = min((uint32_t)IDB_TILE, q.e - t0);
+            idb_tile t;
+            t.q0 = q0;
+            t.t0 = t0;
+            t.bucket = bucket;
+            t.shape = qn | (tn << 8) | ((J == q.I ? 1u : 0u) << 16);
+            tiles[o++] = t;
+        }
+    }
+}
+
+__global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int32_t *core, uint32_t *small)
+{
+    uint32_t local = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t c = cnt[i] >= minpts;
+        core[i] = c;
+        local += (uint32_t)c;
+    }
+    if (local) atomicAdd(small + 9, local);
+}
+
+__device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t x)
+{
+    uint32_t p = par[x];
+    while (p != x) {
+        x = p;
+        p = par[x];
+    }
+    return x;
+}
+
+// R9/R10: label root of every unique row (its component root, its border
+// core's root, or none) and root flags for R12.
+__global__ void k_roots(const uint32_t *uniq, const uint32_t *num, const int32_t *core, const uint32_t *par,
+                        const uint32_t *bmin, uint32_t *lab, uint32_t *rflag)
+{
+    const uint32_t U = num[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < U; k += gridDim.x * blockDim.x) {
+        const uint32_t u = uniq[k];
+        uint32_t l = IDB_NONE;
+        if (core[u]) {
+            l = find_root(par, u);
+            if (l == u) rflag[u] = 1u;
+        } else if (bmin[u] != IDB_NONE) {
+            l = find_root(par, bmin[u]);
+        }
+        lab[u] = l;
+    }
+}
+
+__global__ void k_tree_depth(const uint32_t *rep, const int32_t *core, const uint32_t *par, const uint32_t *bmin,
+                             uint32_t n, uint32_t *hist)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t d = 0, x = i;
+        if (rep[x] != x) { x = rep[x]; ++d; }
+        if (!core[x] && bmin[x] != IDB_NONE) { x = bmin[x]; ++d; }
+        if (core[x])
+            while (par[x] != x && d < 50) { x = par[x]; ++d; }
+        atomicAdd(hist + (d > 50 ? 50 : d), 1u);
+    }
+}
+
+// R11/R12 labels plus the per-row outputs.
+__global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_t *rank, const uint32_t *rid,
+                         const uint32_t *rstart, const int32_t *core, const uint32_t *copies, const int32_t *cnt,
+                         uint32_t n, uint32_t *cid, int32_t *core_out, uint32_t *cop_out, uint32_t *cnt_out,
+                         uint32_t *small)
+{
+    uint32_t noise = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t l = lab[rep[i]];
+        uint32_t c = 0;
+        if (l != IDB_NONE) {
+            const uint32_t base = rid ? rank[rstart[rid[i]]] : 0u;
+            c = rank[l] - base + 1u;
+        }
+        noise += c == 0;
+        cid[i] = c;
+        core_out[i] = core[i];
+        cop_out[i] = copies[i];
+        if (cnt_out) cnt_out[i] = (uint32_t)cnt[i];
+    }
+    if (noise) atomicAdd(small + 10, noise);
+}
+
+// First unique position of every repertoire (binary search over uniques).
+__global__ void k_ustart(const uint32_t *uniq, const uint32_t *num, const uint32_t *rstart, uint32_t n_reps,
+                         uint32_t *ustart)
+{
+    const uint32_t U = num[0];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= n_reps; r += gridDim.x * blockDim.x) {
+        const uint32_t v = rstart[r];
+        uint32_t lo = 0, hi = U;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (uniq[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        ustart[r] = lo;
+    }
+}
+
+__global__ void k_uniques_out(const uint32_t *uniq, const uint32_t *num, const uint32_t *rid, const uint32_t *rstart,
+                              const uint32_t *ustart, uint32_t *out)
+{
+    const uint32_t U = num[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < U; k += gridDim.x * blockDim.x) {
+        const uint32_t u = uniq[k];
+        if (!rid) {
+            out[k] = u;
+        } else {
+            const uint32_t r = rid[u];
+            out[rstart[r] + (k - ustart[r])] = u - rstart[r];
+        }
+    }
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t bs = 256)
+{
+    uint64_t b = (n + bs - 1) / bs;
+    if (b > 65535u) b = 65535u;
+    return b ? (uint32_t)b : 1u;
+}
+
+// ---------------------------------------------------------------------------
+// rocPRIM wrappers (temporary storage from the arena)
+void sort_pairs_u64(idb_ctx *c, uint64_t *kin, uint64_t *kout, uint32_t *vin, uint32_t *vout, size_t n, unsigned end_bit)
+{
+    size_t bytes = 0;
+    check(c, rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, end_bit, c->stream), "radix sort size");
+    void *t = tmp(c, bytes);
+    check(c, rocprim::radix_sort_pairs(t, bytes, kin, kout, vin, vout, n, 0u, end_bit, c->stream), "radix sort");
+}
+
+template <typename T>
+void exclusive_scan_total(idb_ctx *c, const T *in, T *out, size_t n_plus_1)
+{
+    size_t bytes = 0;
+    check(c, rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), n_plus_1, rocprim::plus<T>(), c->stream), "scan size");
+    void *t = tmp(c, bytes);
+    check(c, rocprim::exclusive_scan(t, bytes, in, out, T(0), n_plus_1, rocprim::plus<T>(), c->stream), "scan");
+}
+
+void max_scan(idb_ctx *c, const uint32_t *in, uint32_t *out, size_t n)
+{
+    size_t bytes = 0;
+    check(c, rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::maximum<uint32_t>(), c->stream), "max scan size");
+    void *t = tmp(c, bytes);
+    check(c, rocprim::inclusive_scan(t, bytes, in, out, n, rocprim::maximum<uint32_t>(), c->stream), "max scan");
+}
+
+void select_flagged(idb_ctx *c, size_t n, const uint8_t *flags, uint32_t *out, uint32_t *count)
+{
+    rocprim::counting_iterator<uint32_t> it(0u);
+    size_t bytes = 0;
+    check(c, rocprim::select(nullptr, bytes, it, flags, out, count, n, c->stream), "select size");
+    void *t = tmp(c, bytes);
+    check(c, rocprim::select(t, bytes, it, flags, out, count, n, c->stream), "select");
+}
+
+void readback(idb_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "readback");
+    check(c, hipStreamSynchronize(c->stream), "readback sync");
+}
+
+template <typename T>
+const T *stage_in(idb_ctx *c, Slot s, const T *src, size_t count, uint32_t mem)
+{
+    if (mem == IDB_MEM_DEVICE || count == 0) return src;
+    T *d = buf<T>(c, s, count);
+    check(c, hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream), "input copy");
+    return d;
+}
+
+unsigned bits_for(uint64_t v)
+{
+    unsigned b = 0;
+    while (b < 64 && (v >> b)) ++b;
+    return b;
+}
+
+void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "event record"); }
+
+// ---------------------------------------------------------------------------
+// The pipeline.  rstart (host) has n_reps + 1 row offsets; n_reps == 0 means a
+// single repertoire without repertoire ids.
+void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_reps, const idb_params *p,
+         idb_outputs *o, uint32_t *n_uniques_h, idb_stats *st)
+{
+    const uint32_t N = d->n_rows;
+    const uint32_t vw = d->v_width, jw = d->j_width;
+    if (o->mem != IDB_MEM_HOST && o->mem != IDB_MEM_DEVICE) fail(c, IDB_ERR_ARG, "outputs.mem must be host or device");
+    if (d->mem != IDB_MEM_HOST && d->mem != IDB_MEM_DEVICE) fail(c, IDB_ERR_ARG, "dataset.mem must be host or device");
+    if (!o->clone_id || !o->core || !o->copies || !o->uniques) fail(c, IDB_ERR_ARG, "missing output array");
+    if (N && (!d->off || !d->len || (d->seq_bytes && !d->seq))) fail(c, IDB_ERR_ARG, "missing input array");
+    if (N && ((vw && !d->vbits) || (jw && !d->jbits))) fail(c, IDB_ERR_ARG, "missing bitset array");
+    if (vw > 64 || jw > 64) fail(c, IDB_ERR_UNSUPPORTED, "gene bitsets wider than 64 words (2048 genes)");
+    if (p->debug_hash_bits > 64) fail(c, IDB_ERR_ARG, "debug_hash_bits must be 0..64");
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_rows = N;
+    }
+    if (N == 0) {
+        if (n_uniques_h)
+            for (uint32_t r = 0; r < n_reps; ++r) n_uniques_h[r] = 0;
+        return;
+    }
+    hipStream_t s = c->stream;
+    record(c, 0);
+
+    // inputs
+    const uint8_t *seq = stage_in<uint8_t>(c, S_SEQ, d->seq, d->seq_bytes, d->mem);
+    const uint64_t *off = stage_in<uint64_t>(c, S_OFF, d->off, N, d->mem);
+    const uint32_t *len = stage_in<uint32_t>(c, S_LEN, d->len, N, d->mem);
+    const int32_t *vb = stage_in<int32_t>(c, S_V, d->vbits, (size_t)N * vw, d->mem);
+    const int32_t *jb = stage_in<int32_t>(c, S_J, d->jbits, (size_t)N * jw, d->mem);
+    if (!seq) seq = buf<uint8_t>(c, S_SEQ, 4);
+    if (!vb) vb = buf<int32_t>(c, S_V, 1);
+    if (!jb) jb = buf<int32_t>(c, S_J, 1);
+    uint32_t *rid = nullptr, *rstart = nullptr;
+    if (n_reps > 0) {
+        rstart = buf<uint32_t>(c, S_RSTART, n_reps + 1);
+        check(c, hipMemcpyAsync(rstart, rstart_h, sizeof(uint32_t) * (n_reps + 1), hipMemcpyHostToDevice, s), "rstart");
+        rid = buf<uint32_t>(c, S_RID, N);
+        hipLaunchKernelGGL(k_fill_rid, dim3(blocks_for(N)), dim3(256), 0, s, rstart, n_reps, N, rid);
+    }
+
+    // --- Identify Duplicates
+    uint32_t *small = buf<uint32_t>(c, S_SMALL, 128);
+    check(c, hipMemsetAsync(small, 0, 128 * sizeof(uint32_t), s), "memset");
+    uint64_t *key = buf<uint64_t>(c, S_KEY, N), *key2 = buf<uint64_t>(c, S_KEY2, N);
+    uint32_t *idx = buf<uint32_t>(c, S_IDX, N), *idx2 = buf<uint32_t>(c, S_IDX2, N);
+    const uint64_t hmask = (p->debug_hash_bits == 0 || p->debug_hash_bits >= 64) ? ~0ull
+                                                                                 : ((1ull << p->debug_hash_bits) - 1ull);
+    hipLaunchKernelGGL(k_rowscan, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, N, hmask,
+                       key, idx, small);
+    uint32_t hs[16];
+    readback(c, hs, small, sizeof(hs));
+    const uint32_t maxlen = hs[8];
+    uint8_t cmap_h[256];
+    uint32_t K = 0;
+    for (uint32_t b = 0; b < 256; ++b) {
+        cmap_h[b] = (uint8_t)K;
+        if (hs[b >> 5] >> (b & 31) & 1u) ++K;
+    }
+    if (maxlen > 2048) fail(c, IDB_ERR_UNSUPPORTED, "sequences longer than 2048 letters are not supported");
+    const bool ham = p->use_hamming != 0;
+    uint32_t W = 0, S = 0, HW = 0, bits = 0;
+    if (ham) {
+        bits = K <= 4 ? 2u : 8u;
+        HW = std::max(1u, (maxlen * bits + 31) / 32);
+        if (HW > IDB_MAX_HW)
+            fail(c, IDB_ERR_UNSUPPORTED, "Hamming rows longer than " + std::to_string(IDB_MAX_HW * 32 / bits) +
+                                             " letters are not supported for this alphabet");
+    } else {
+        W = std::max(1u, (maxlen + 31) / 32);
+        if (W > IDB_MAX_W) fail(c, IDB_ERR_UNSUPPORTED, "Levenshtein rows longer than 256 letters are not supported");
+        if ((size_t)K * W * 64 * 4 > 65536)
+            fail(c, IDB_ERR_UNSUPPORTED, "alphabet of " + std::to_string(K) + " letters with rows of " +
+                                             std::to_string(maxlen) + " letters exceeds the LDS budget");
+        S = std::max(4u, (maxlen + 3) / 4 * 4);
+    }
+    sort_pairs_u64(c, key, key2, idx, idx2, N, 64);
+    uint32_t *head0 = buf<uint32_t>(c, S_HEAD, N);
+    uint32_t *rep = buf<uint32_t>(c, S_REP, N);
+    uint32_t *copies = buf<uint32_t>(c, S_COPIES, N);
+    hipLaunchKernelGGL(k_seghead, dim3(blocks_for(N)), dim3(256), 0, s, key2, N, idx);  // idx reused as scratch
+    max_scan(c, idx, head0, N);
+    check(c, hipMemsetAsync(copies, 0, sizeof(uint32_t) * N, s), "memset");
+    hipLaunchKernelGGL(k_dedup, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, idx2, head0, N,
+                       rep, copies);
+    record(c, 1);
+
+    // --- Determine copies: uniques, count init
+    uint8_t *flag = buf<uint8_t>(c, S_FLAG, N);
+    int32_t *cnt = buf<int32_t>(c, S_CNT, N);
+    uint32_t *par = buf<uint32_t>(c, S_PAR, N), *bmin = buf<uint32_t>(c, S_BMIN, N);
+    uint32_t *rflag = buf<uint32_t>(c, S_RFLAG, N + 1);
+    uint32_t *rank = buf<uint32_t>(c, S_RANK, N + 1);
+    hipLaunchKernelGGL(k_after_dedup, dim3(blocks_for(N)), dim3(256), 0, s, rep, copies, N, flag, cnt, par, bmin, rflag);
+    uint32_t *uniq = buf<uint32_t>(c, S_UNIQ, N);
+    uint32_t *num = buf<uint32_t>(c, S_NUM, 4);
+    select_flagged(c, N, flag, uniq, num);
+    record(c, 2);
+
+    // --- Identify core points: buckets, tiles, count pass
+    const uint32_t gj = 32 * std::max(jw, 1u);
+    const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
+    uint32_t U = 0;
+    readback(c, &U, num, sizeof(uint32_t));
+    uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)U + 1);
+    uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)U + 1);
+    uint32_t single = 0;
+    uint32_t E = 0;
+    // With no V or no J genes the gate of R6 fails for every pair: no entries.
+    if (vw > 0 && jw > 0) {
+        hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)U + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb, jw,
+                           0u, memb);
+        exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)U + 1);
+        uint64_t E64 = 0;
+        readback(c, &E64, eoff + U, sizeof(uint64_t));
+        // Rows with many genes are copied into many buckets; past 4x the
+        // uniques one bucket per repertoire with an explicit gate is cheaper.
+        if (E64 > 4ull * U + 4096 || gvj64 * std::max(n_reps, 1u) >= (1ull << 47)) {
+            single = 1;
+            hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)U + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb,
+                               jw, 1u, memb);
+            exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)U + 1);
+            E64 = U;
+        }
+        E = (uint32_t)E64;
+    }
+    const uint32_t gvj = single ? 1u : (uint32_t)std::min<uint64_t>(gvj64, 0xFFFFFFFFull);
+    // threshold tables: count pass, fuse pass, band end
+    std::vector<int16_t> tc(maxlen + 1), tf(maxlen + 1), hl(maxlen + 1);
+    for (uint32_t L = 0; L <= maxlen; ++L) {
+        const int64_t t = threshold_of(L, *p);
+        if (ham) {
+            tc[L] = clamp16(t < 0 ? 0 : t);   // count pass compares only after a mismatch
+            tf[L] = clamp16(t);               // fuse pass compares after every letter
+        } else {
+            tc[L] = tf[L] = clamp16(t);
+        }
+    }
+    bool banded = true;
+    if (!ham)
+        for (uint32_t L = 1; L <= maxlen; ++L)
+            if ((int64_t)L - tc[L] < (int64_t)(L - 1) - tc[L - 1]) banded = false;
+    for (uint32_t l = 0; l <= maxlen; ++l) {
+        uint32_t h = l;
+        if (!ham) {
+            h = 0;
+            for (uint32_t L = 0; L <= maxlen; ++L)
+                if ((int64_t)L - tc[L] <= (int64_t)l) h = L;
+        }
+        hl[l] = (int16_t)std::min<uint32_t>(h, 32767u);
+    }
+    int16_t *tlut_c = buf<int16_t>(c, S_TLUT_C, maxlen + 1), *tlut_f = buf<int16_t>(c, S_TLUT_F, maxlen + 1);
+    uint16_t *hilen = buf<uint16_t>(c, S_HILEN, maxlen + 1);
+    uint8_t *cmap = buf<uint8_t>(c, S_CMAP, 256);
+    check(c, hipMemcpyAsync(tlut_c, tc.data(), sizeof(int16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
+    check(c, hipMemcpyAsync(tlut_f, tf.data(), sizeof(int16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
+    check(c, hipMemcpyAsync(hilen, hl.data(), sizeof(uint16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
+    check(c, hipMemcpyAsync(cmap, cmap_h, 256, hipMemcpyHostToDevice, s), "cmap");
+
+    unsigned long long *work = buf<unsigned long long>(c, S_WORK, 4);
+    check(c, hipMemsetAsync(work, 0, 4 * sizeof(unsigned long long), s), "memset");
+    uint32_t B = 0, T = 0;
+    idb_pair_args a{};
+    if (E > 0) {
+        uint64_t *ekey = buf<uint64_t>(c, S_EKEY, E), *ekey2 = buf<uint64_t>(c, S_EKEY2, E);
+        uint32_t *erow = buf<uint32_t>(c, S_EROW, E), *erow2 = buf<uint32_t>(c, S_EROW2, E);
+        hipLaunchKernelGGL(k_emit_entries, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, eoff, vb, vw, jb, jw, len, rid,
+                           single, gj, gvj, ekey, erow);
+        const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
+        sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, 16 + bits_for(max_bucket));
+        uint16_t *elen = buf<uint16_t>(c, S_ELEN, E);
+        uint32_t *ecp = buf<uint32_t>(c, S_ECP, E);
+        uint8_t *echk = buf<uint8_t>(c, S_ECHK, E);
+        uint8_t *bflag = buf<uint8_t>(c, S_BFLAG, E);
+        uint32_t *ecodes = ham ? nullptr : buf<uint32_t>(c, S_ECODES, (size_t)E * (S / 4));
+        uint32_t *epk = ham ? buf<uint32_t>(c, S_EPK, (size_t)E * HW) : nullptr;
+        hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
+                           jb, jw, single, gj, gvj, cmap, S, HW, bits, elen, ecp, echk, ecodes, epk, bflag);
+        uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
+        select_flagged(c, E, bflag, bstart, num + 1);
+        uint32_t *qbcnt = buf<uint32_t>(c, S_QBCNT, (size_t)E + 1);
+        uint32_t *qboff = buf<uint32_t>(c, S_QBOFF, (size_t)E + 1);
+        // bucket count B is read on the device; scans need the length, so read it
+        readback(c, &B, num + 1, sizeof(uint32_t));
+        hipLaunchKernelGGL(k_bucket_blocks, dim3(blocks_for((uint64_t)B + 1)), dim3(256), 0, s, bstart, num + 1, E, qbcnt);
+        exclusive_scan_total<uint32_t>(c, qbcnt, qboff, (size_t)B + 1);
+        uint32_t QB = 0;
+        readback(c, &QB, qboff + B, sizeof(uint32_t));
+        uint32_t *tcount = buf<uint32_t>(c, S_TCNT, (size_t)QB + 1);
+        uint32_t *toff = buf<uint32_t>(c, S_TOFF, (size_t)QB + 1);
+        hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for((uint64_t)QB + 1)), dim3(256), 0, s, qboff, num + 1, bstart, E,
+                           elen, hilen, banded ? 1u : 0u, tcount);
+        exclusive_scan_total<uint32_t>(c, tcount, toff, (size_t)QB + 1);
+        readback(c, &T, toff + QB, sizeof(uint32_t));
+        idb_tile *tiles = buf<idb_tile>(c, S_TILES, std::max<uint32_t>(T, 1u));
+        hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(QB)), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
+                           banded ? 1u : 0u, ekey2, gvj, toff, tiles);
+        a.tiles = tiles;
+        a.n_tiles = T;
+        a.e_row = erow2;
+        a.e_len = elen;
+        a.e_cp = ecp;
+        a.e_chk = echk;
+        a.e_codes = (const uint8_t *)ecodes;
+        a.e_pk = epk;
+        a.S = S;
+        a.HW = HW;
+        a.K = std::max(K, 1u);
+        a.bits = bits;
+        a.vbits = vb;
+        a.jbits = jb;
+        a.vw = vw;
+        a.jw = jw;
+        a.gj = gj;
+        a.gvj = gvj;
+        a.single_bucket = single;
+        a.tlut = tlut_c;
+        a.cnt = cnt;
+        a.work = work;
+        check(c, hipEventRecord(c->kev[0], s), "event record");
+        check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
+        check(c, hipEventRecord(c->kev[1], s), "event record");
+    }
+    int32_t *core = buf<int32_t>(c, S_CORE, N);
+    hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, small);
+    record(c, 3);
+
+    // --- Grow clusters
+    if (E > 0) {
+        a.tlut = tlut_f;
+        a.core = core;
+        a.par = par;
+        a.bmin = bmin;
+        a.cnt = nullptr;
+        check(c, hipEventRecord(c->kev[2], s), "event record");
+        check(c, ham ? idb_launch_fuse_hamming(a, s) : idb_launch_fuse_myers(a, W, s), "fuse kernel launch");
+        check(c, hipEventRecord(c->kev[3], s), "event record");
+    }
+    record(c, 4);
+
+    // --- Reduction of cluster information tree structure
+    uint32_t *lab = buf<uint32_t>(c, S_LAB, N);
+    hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag);
+    uint32_t *hist = nullptr;
+    if (p->tree_stats) {
+        hist = small + 16;
+        hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, hist);
+    }
+    record(c, 5);
+
+    // --- Re-number clusters and identify noise
+    check(c, hipMemsetAsync(rflag + N, 0, sizeof(uint32_t), s), "memset");
+    exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);
+    const bool dev_out = o->mem == IDB_MEM_DEVICE;
+    uint32_t *cid = dev_out ? o->clone_id : buf<uint32_t>(c, S_OUT_CID, N);
+    int32_t *core_o = dev_out ? o->core : buf<int32_t>(c, S_OUT_CORE, N);
+    uint32_t *cop_o = dev_out ? o->copies : buf<uint32_t>(c, S_OUT_COP, N);
+    uint32_t *unq_o = dev_out ? o->uniques : buf<uint32_t>(c, S_OUT_UNQ, N);
+    uint32_t *cnt_o = o->neighbour_count ? (dev_out ? o->neighbour_count : buf<uint32_t>(c, S_OUT_CNT, N)) : nullptr;
+    hipLaunchKernelGGL(k_labels, dim3(blocks_for(N)), dim3(256), 0, s, rep, lab, rank, rid, rstart, core, copies, cnt, N,
+                       cid, core_o, cop_o, cnt_o, small);
+    check(c, hipMemsetAsync(unq_o, 0, sizeof(uint32_t) * N, s), "memset");
+    uint32_t *ustart = nullptr;
+    if (n_reps > 0) {
+        ustart = buf<uint32_t>(c, S_USTART, n_reps + 1);
+        hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart, n_reps,
+                           ustart);
+    }
+    hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart, unq_o);
+    record(c, 6);
+    if (!dev_out) {
+        check(c, hipMemcpyAsync(o->clone_id, cid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        check(c, hipMemcpyAsync(o->core, core_o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        check(c, hipMemcpyAsync(o->copies, cop_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        check(c, hipMemcpyAsync(o->uniques, unq_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        if (cnt_o)
+            check(c, hipMemcpyAsync(o->neighbour_count, cnt_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s),
+                  "output copy");
+    }
+    record(c, 7);
+    std::vector<uint32_t> us;
+    if (n_reps > 0) {
+        us.resize(n_reps + 1);
+        check(c, hipMemcpyAsync(us.data(), ustart, sizeof(uint32_t) * (n_reps + 1), hipMemcpyDeviceToHost, s), "ustart");
+    }
+    uint32_t fin[128];
+    unsigned long long wk[4];
+    check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
+    check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
+    check(c, hipStreamSynchronize(s), "final sync");
+    if (n_uniques_h)
+        for (uint32_t r = 0; r < n_reps; ++r) n_uniques_h[r] = us[r + 1] - us[r];
+    if (st) {
+        st->n_uniques = U;
+        st->n_cores = fin[9];
+        st->n_noise = fin[10];
+        st->max_len = maxlen;
+        st->alphabet_size = K;
+        st->n_buckets = B;
+        st->bucket_entries = E;
+        st->tiles = T;
+        st->gated_pairs = wk[0];
+        st->neighbour_pairs = wk[1];
+        st->fuse_pairs = wk[2];
+        st->gated_letters = wk[3];
+        if (E > 0 && T > 0) {
+            (void)hipEventElapsedTime(&st->count_kernel_ms, c->kev[0], c->kev[1]);
+            (void)hipEventElapsedTime(&st->fuse_kernel_ms, c->kev[2], c->kev[3]);
+        }
+        if (p->tree_stats)
+            for (int k = 0; k <= 50; ++k) st->tree_depth[k] = fin[16 + k];
+        // Stage slots follow the reference's timers; duplicate-tree reduction
+        // is part of the duplicate stage here (representatives are direct).
+        const int from[8] = {0, 1, 1, 2, 3, 4, 5, 0}, to[8] = {1, 1, 2, 3, 4, 5, 6, 7};
+        for (int k = 0; k < 8; ++k) {
+            float ms = 0.f;
+            if (from[k] != to[k]) (void)hipEventElapsedTime(&ms, c->ev[from[k]], c->ev[to[k]]);
+            st->stage_ms[k] = ms;
+        }
+    }
+    if (st) {
+        uint32_t ncl = 0;
+        check(c, hipMemcpy(&ncl, rank + N, sizeof(uint32_t), hipMemcpyDeviceToHost), "stats");
+        st->n_clusters = ncl;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+int idb_create(idb_ctx **out, int device)
+{
+    if (!out) return IDB_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return IDB_ERR_HIP;
+    }
+    idb_ctx *c = new idb_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return IDB_ERR_HIP;
+    }
+    for (int k = 0; k <= IDB_N_STAGES; ++k)
+        if (hipEventCreate(&c->ev[k]) != hipSuccess) {
+            delete c;
+            return IDB_ERR_HIP;
+        }
+    for (int k = 0; k < 4; ++k)
+        if (hipEventCreate(&c->kev[k]) != hipSuccess) {
+            delete c;
+            return IDB_ERR_HIP;
+        }
+    *out = c;
+    return IDB_OK;
+}
+
+void idb_destroy(idb_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int s = 0; s < S_COUNT; ++s)
+        if (c->ptr[s]) (void)hipFree(c->ptr[s]);
+    for (int k = 0; k <= IDB_N_STAGES; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 4; ++k)
+        if (c->kev[k]) (void)hipEventDestroy(c->kev[k]);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *idb_last_error(const idb_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int idb_gene_bitsets(uint32_t n, const uint8_t *calls, const uint64_t *call_off, const uint32_t *call_len,
+                     uint32_t n_genes, const uint8_t *genes, const uint64_t *gene_off, const uint32_t *gene_len,
+                     int32_t *out, uint32_t width)
+{
+    if ((n && (!call_off || !call_len || !out)) || (n_genes && (!gene_off || !gene_len)))
+        return IDB_ERR_ARG;
+    if (width < (n_genes + 31) / 32) return IDB_ERR_ARG;
+    std::memset(out, 0, sizeof(int32_t) * (size_t)n * width);
+    for (uint32_t g = 0; g < n_genes; ++g) {
+        const uint32_t gl = gene_len[g];
+        // The reference launches one block of len(gene) threads per gene
+        // (cuda/BitsetVector.inl:204); 0 or more than 1024 threads fails the
+        // launch silently and the bit stays clear.
+        if (gl == 0 || gl > 1024) continue;
+        const std::string_view needle((const char *)genes + gene_off[g], gl);
+        for (uint32_t i = 0; i < n; ++i) {
+            const std::string_view hay((const char *)calls + call_off[i], call_len[i]);
+            if (hay.find(needle) != std::string_view::npos) out[(size_t)i * width + (g >> 5)] |= (int32_t)(1u << (g & 31));
+        }
+    }
+    return IDB_OK;
+}
+
+static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uint32_t nr, const idb_params *p,
+                       idb_outputs *o, uint32_t *nu, idb_stats *st)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!d || !p || !o) {
+        c->err = "null argument";
+        return IDB_ERR_ARG;
+    }
+    c->err.clear();
+    try {
+        check(c, hipSetDevice(c->device), "hipSetDevice");
+        run(c, d, rs, nr, p, o, nu, st);
+    } catch (const Fail &f) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(c->stream);
+        return f.code;
+    } catch (const std::exception &e) {
+        c->err = e.what();
+        return IDB_ERR_HIP;
+    }
+    return IDB_OK;
+}
+
+int idb_dbscan(idb_ctx *c, const idb_dataset *d, const idb_params *p, idb_outputs *o, idb_stats *st)
+{
+    return run_guarded(c, d, nullptr, 0, p, o, nullptr, st);
+}
+
+int idb_dbscan_batch(idb_ctx *c, const idb_dataset *d, const uint32_t *rep_start, uint32_t n_reps, const idb_params *p,
+                     idb_outputs *o, uint32_t *n_uniques, idb_stats *st)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!rep_start || !n_uniques || n_reps == 0 || !d) {
+        c->err = "batch needs rep_start, n_uniques and n_reps > 0";
+        return IDB_ERR_ARG;
+    }
+    if (rep_start[0] != 0 || rep_start[n_reps] != d->n_rows) {
+        c->err = "rep_start must run from 0 to n_rows";
+        return IDB_ERR_ARG;
+    }
+    for (uint32_t r = 0; r < n_reps; ++r)
+        if (rep_start[r + 1] < rep_start[r]) {
+            c->err = "rep_start must be non-decreasing";
+            return IDB_ERR_ARG;
+        }
+    return run_guarded(c, d, rep_start, n_reps, p, o, n_uniques, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// idb_internal.h — device data layout and launch interfaces shared by the HIP
+// translation units of libidb_hip.so.
+//
+// Layout in HBM for one run (all arrays device-resident, row = input row):
+//   rows:     seq bytes, off[u64], len[u32], vbits[n*vw], jbits[n*jw]  (inputs)
+//             key[u64] / idx[u32]  duplicate hash and its sort permutation
+//             rep[u32]  representative (min index of the duplicate class)
+//             copies[u32], cnt[i32]  class size / exact neighbour count
+//             par[u32]  union-find parent over rows, bmin[u32] border claim
+//   entries:  one per (unique row, V gene, J gene) bucket membership, sorted by
+//             (repertoire, V gene, J gene, length, row); a bucket is a
+//             contiguous run.  Per entry: row, length, copies, check flag and
+//             the row's letters as dense 8-bit codes (stride S bytes) or, for
+//             Hamming mode, packed words (stride HW words).
+//   tiles:    (query block, target block) pairs of <= 64 entries each inside a
+//             bucket whose length band can hold a neighbour pair.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define IDB_NONE 0xFFFFFFFFu
+#define IDB_TILE 64               // entries per query / target block (one wave)
+#define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
+#define IDB_MAX_HW 64             // Hamming packed words per row
+#define IDB_MAX_LEN 65535u
+
+struct idb_tile {
+    uint32_t q0;                  // first query entry
+    uint32_t t0;                  // first target entry
+    uint32_t bucket;              // bucket id (gene pair) for the canonical check
+    uint32_t shape;               // qn | tn << 8 | diag << 16
+};
+
+// Uniform parameters of a pair-kernel launch.
+struct idb_pair_args {
+    const idb_tile *tiles;
+    uint32_t n_tiles;
+    const uint32_t *e_row;
+    const uint16_t *e_len;
+    const uint32_t *e_cp;         // copies of the entry's row
+    const uint8_t *e_chk;         // 1 when the row may share a lower gene
+    const uint8_t *e_codes;       // Myers: stride S bytes of codes
+    const uint32_t *e_pk;         // Hamming: stride HW packed words
+    uint32_t S, HW, K;
+    uint32_t bits;                // Hamming code width: 2 or 8
+    const int32_t *vbits, *jbits;
+    uint32_t vw, jw;
+    uint32_t gj;                  // J genes per bucket row (32*jw)
+    uint32_t gvj;                 // genes pairs per repertoire (32*vw*gj)
+    uint32_t single_bucket;       // 1: every pair is gated explicitly
+    const int16_t *tlut;          // threshold by max length (count or fuse pass)
+    int32_t *cnt;                 // count pass output
+    const int32_t *core;          // fuse pass input (per row)
+    uint32_t *par;                // fuse pass: union-find parents
+    uint32_t *bmin;               // fuse pass: border claims
+    unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] text letters
+};
+
+// Kernel launchers (defined in idb_pairs.hip).
+hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
+hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
+hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s);
+hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s);

@@ -1,0 +1,362 @@
+// idb_pairs.hip — the O(pairs) kernels: neighbour counting (R8) and cluster
+// fusion (R9/R10) over bucket tiles, for gfx950.
+//
+// Replaces the reference's pair kernels half_core_points / fuse_clusters
+// (Levenshtein, cuda/PSDBSCAN.hcu:751-931) and half_hamming_core_points /
+// fuse_hamming_clusters (Hamming, :585-742).  The reference evaluates every
+// unique pair, one block per pair in Levenshtein mode with a barrier per DP
+// anti-diagonal.  Here a wave owns one tile of <= 64 target entries x <= 64
+// query entries of one V/J bucket:
+//   * each lane holds one target (the Myers pattern) and builds its match
+//     masks Peq[letter][word] in its own LDS column ([letter][word][lane]
+//     layout, so a wave's 64 lanes hit 64 consecutive dwords: no bank
+//     conflicts);
+//   * the queries are the Myers text and are wave-uniform: their letters come
+//     through scalar loads, and the wave walks them one after another;
+//   * gate work per pair is the length band, the triangle order inside the
+//     diagonal tile and, only for rows with ambiguous calls, the canonical
+//     bucket test (lowest common V gene and J gene equal the bucket's), so
+//     each unordered pair is evaluated in exactly one bucket.
+// Count pass: copies-weighted counts, accumulated in registers and flushed
+// with one atomic per target per tile and one per query hit.  Fuse pass: pairs
+// with at least one core; core-core pairs are joined in a lock-free
+// union-find that links the larger root under the smaller (roots are
+// component minima), and a non-core row records its minimum-index core
+// neighbour with atomicMin (the canonical form of the reference's first-come
+// atomicCAS at :113-122).
+#include "idb_internal.h"
+#include "pairdist.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint32_t uf_find(uint32_t *par, uint32_t x)
+{
+    uint32_t p = ld_agent(par + x);
+    while (p != x) {
+        const uint32_t gp = ld_agent(par + p);
+        if (gp != p) {
+            // path halving: gp is an ancestor of x, so the shortcut is safe
+            __hip_atomic_store(par + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+// Link the larger root under the smaller one.  Every retry continues from a
+// strictly smaller index, so the loop ends.
+__device__ void uf_union(uint32_t *par, uint32_t a, uint32_t b)
+{
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    while (a != b) {
+        if (a < b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        uint32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(par + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        a = uf_find(par, expected);
+        b = uf_find(par, b);
+    }
+}
+
+__device__ __forceinline__ int lowest_common(const int32_t *a, const int32_t *b, uint32_t w)
+{
+    for (uint32_t k = 0; k < w; ++k) {
+        const uint32_t m = (uint32_t)(a[k] & b[k]);
+        if (m) return (int)(k * 32 + __builtin_ctz(m));
+    }
+    return -1;
+}
+
+// Canonical-bucket test for a pair of rows that both carry more than the
+// bucket's genes (or every pair, in single-bucket mode, where it is the V/J
+// gate of cuda/BitsetVector.inl:214-237).
+__device__ bool pair_in_bucket(const idb_pair_args &a, uint32_t x, uint32_t y, uint32_t bucket)
+{
+    const int32_t *vx = a.vbits + (size_t)x * a.vw, *vy = a.vbits + (size_t)y * a.vw;
+    const int32_t *jx = a.jbits + (size_t)x * a.jw, *jy = a.jbits + (size_t)y * a.jw;
+    const int lv = lowest_common(vx, vy, a.vw), lj = lowest_common(jx, jy, a.jw);
+    if (a.single_bucket) return lv >= 0 && lj >= 0;
+    const uint32_t bl = bucket % a.gvj;
+    return lv == (int)(bl / a.gj) && lj == (int)(bl % a.gj);
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
+// walks a contiguous run of tiles (neighbouring tiles share query blocks,
+// which then stay in that XCD's L2).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+template <int W, bool FUSE>
+__global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t K = a.K;
+    uint32_t *peq = lds + (size_t)wib * K * W * 64;
+    const uint32_t total_waves = gridDim.x * wpb;
+    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
+    unsigned long long gated = 0, hits = 0, letters = 0;
+
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
+        const idb_tile tile = a.tiles[ti];
+        const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
+        const bool diag = (tile.shape >> 16) & 1u;
+        const bool tvalid = lane < tn;
+        const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
+        uint32_t y = 0, m = 0, cy = 0;
+        bool chk_y = false;
+        int32_t core_y = 0;
+        if (tvalid) {
+            y = a.e_row[p];
+            m = a.e_len[p];
+            cy = a.e_cp[p];
+            chk_y = a.e_chk[p] != 0;
+            if (FUSE) core_y = a.core[y];
+        }
+        // Peq for this lane's pattern, in its own LDS column.
+        for (uint32_t c = 0; c < K; ++c)
+#pragma unroll
+            for (int w = 0; w < W; ++w) peq[(c * W + w) * 64 + lane] = 0u;
+        if (tvalid) {
+            const uint32_t *codes = (const uint32_t *)(a.e_codes + (size_t)p * a.S);
+            for (uint32_t i = 0; i < m; i += 4) {
+                const uint32_t word = codes[i >> 2];
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k)
+                    if (i + k < m) {
+                        const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                        peq[(c * W + ((i + k) >> 5)) * 64 + lane] |= 1u << ((i + k) & 31);
+                    }
+            }
+        }
+        const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
+        int32_t acc_y = 0;
+
+        for (uint32_t qi = 0; qi < qn; ++qi) {
+            const uint32_t q = tile.q0 + qi;
+            const uint32_t x = a.e_row[q];
+            const uint32_t lx = a.e_len[q];
+            const bool chk_x = a.e_chk[q] != 0;
+            int32_t core_x = 0;
+            if (FUSE) core_x = a.core[x];
+            const uint32_t L = lx > m ? lx : m;
+            const int32_t t = tvalid ? (int32_t)a.tlut[L] : -1;
+            const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
+            bool valid = tvalid && (!diag || lane > qi) && dl <= t;
+            if (FUSE) valid = valid && (core_x | core_y);
+            if (valid && (a.single_bucket || (chk_x && chk_y))) valid = pair_in_bucket(a, x, y, tile.bucket);
+            const unsigned long long vm = __ballot(valid);
+            if (!vm) continue;
+            gated += __popcll(vm);
+            letters += (unsigned long long)__popcll(vm) * lx;
+
+            idb_myers<W> st;
+            idb_myers_init<W>(st, m);
+            const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
+            for (uint32_t j = 0; j < lx; j += 4) {
+                const uint32_t word = xc[j >> 2];
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) {
+                    if (j + k < lx) {
+                        const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                        uint32_t eq[W];
+#pragma unroll
+                        for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + lane];
+                        idb_myers_step<W>(st, eq, top_w, top_b);
+                    }
+                }
+            }
+            const int32_t dist = m ? st.score : (int32_t)lx;
+            const bool hit = valid && dist <= t;
+            const unsigned long long hm = __ballot(hit);
+            if (!hm) continue;
+            if (!FUSE) {
+                hits += __popcll(hm);
+                if (hit) acc_y += (int32_t)a.e_cp[q];
+                const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
+                if (lane == 0) atomicAdd(a.cnt + x, s);
+            } else if (hit) {
+                if (core_x && core_y) uf_union(a.par, x, y);
+                else if (core_x) atomicMin(a.bmin + y, x);
+                else atomicMin(a.bmin + x, y);
+            }
+        }
+        if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
+    }
+    if (lane == 0 && a.work) {
+        atomicAdd(a.work + (FUSE ? 2 : 0), gated);
+        if (!FUSE) {
+            atomicAdd(a.work + 1, hits);
+            atomicAdd(a.work + 3, letters);
+        }
+    }
+}
+
+template <int HWMAX, int BITS, bool FUSE>
+__global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t wpb)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t total_waves = gridDim.x * wpb;
+    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
+    const uint32_t HW = a.HW;
+    unsigned long long gated = 0, hits = 0;
+
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
+        const idb_tile tile = a.tiles[ti];
+        const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
+        const bool diag = (tile.shape >> 16) & 1u;
+        const bool tvalid = lane < tn;
+        const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
+        uint32_t y = 0, m = 0, cy = 0;
+        bool chk_y = false;
+        int32_t core_y = 0;
+        uint32_t tw[HWMAX];
+#pragma unroll
+        for (int w = 0; w < HWMAX; ++w) tw[w] = 0u;
+        if (tvalid) {
+            y = a.e_row[p];
+            m = a.e_len[p];
+            cy = a.e_cp[p];
+            chk_y = a.e_chk[p] != 0;
+            if (FUSE) core_y = a.core[y];
+            const uint32_t *pk = a.e_pk + (size_t)p * HW;
+#pragma unroll
+            for (int w = 0; w < HWMAX; ++w)
+                if ((uint32_t)w < HW) tw[w] = pk[w];
+        }
+        int32_t acc_y = 0;
+        for (uint32_t qi = 0; qi < qn; ++qi) {
+            const uint32_t q = tile.q0 + qi;
+            const uint32_t x = a.e_row[q];
+            const uint32_t lx = a.e_len[q];
+            const bool chk_x = a.e_chk[q] != 0;
+            int32_t core_x = 0;
+            if (FUSE) core_x = a.core[x];
+            const int32_t t = (int32_t)a.tlut[lx];
+            bool valid = tvalid && (!diag || lane > qi) && lx == m;
+            if (FUSE) valid = valid && (core_x | core_y);
+            if (valid && (a.single_bucket || (chk_x && chk_y))) valid = pair_in_bucket(a, x, y, tile.bucket);
+            const unsigned long long vm = __ballot(valid);
+            if (!vm) continue;
+            gated += __popcll(vm);
+            const uint32_t nw = (lx * BITS + 31) / 32;
+            const uint32_t *xp = a.e_pk + (size_t)q * HW;
+            uint32_t ham = 0;
+#pragma unroll
+            for (int w = 0; w < HWMAX; ++w) {
+                if ((uint32_t)w < nw) {
+                    const uint32_t xw = xp[w];
+                    ham += BITS == 2 ? idb_mismatch2(xw, tw[w]) : idb_mismatch8(xw, tw[w]);
+                }
+            }
+            // a negative fuse-pass threshold fails every non-empty pair
+            const bool hit = valid && ((int32_t)ham <= t || (lx == 0 && t < 0));
+            const unsigned long long hm = __ballot(hit);
+            if (!hm) continue;
+            if (!FUSE) {
+                hits += __popcll(hm);
+                if (hit) acc_y += (int32_t)a.e_cp[q];
+                const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
+                if (lane == 0) atomicAdd(a.cnt + x, s);
+            } else if (hit) {
+                if (core_x && core_y) uf_union(a.par, x, y);
+                else if (core_x) atomicMin(a.bmin + y, x);
+                else atomicMin(a.bmin + x, y);
+            }
+        }
+        if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
+    }
+    if (lane == 0 && a.work) {
+        atomicAdd(a.work + (FUSE ? 2 : 0), gated);
+        if (!FUSE) atomicAdd(a.work + 1, hits);
+    }
+}
+
+uint32_t grid_for(uint32_t n_tiles, uint32_t wpb)
+{
+    // enough waves to fill 256 CUs several times over; each wave then walks
+    // tiles with a stride of the total wave count
+    const uint32_t max_blocks = 256u * 32u / wpb;
+    uint32_t b = (n_tiles + wpb - 1) / wpb;
+    if (b > max_blocks) b = max_blocks;
+    return b ? b : 1u;
+}
+
+template <bool FUSE>
+hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
+{
+    if (a.n_tiles == 0) return hipSuccess;
+    const size_t per_wave = (size_t)a.K * W * 64 * sizeof(uint32_t);
+    uint32_t wpb = (uint32_t)(65536 / per_wave);
+    if (wpb > 4) wpb = 4;
+    if (wpb == 0) return hipErrorInvalidValue;
+    const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
+    const size_t shm = per_wave * wpb;
+    switch (W) {
+    case 1: hipLaunchKernelGGL((k_pairs_myers<1, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 2: hipLaunchKernelGGL((k_pairs_myers<2, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 3: hipLaunchKernelGGL((k_pairs_myers<3, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 4: hipLaunchKernelGGL((k_pairs_myers<4, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 5: hipLaunchKernelGGL((k_pairs_myers<5, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 6: hipLaunchKernelGGL((k_pairs_myers<6, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 7: hipLaunchKernelGGL((k_pairs_myers<7, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 8: hipLaunchKernelGGL((k_pairs_myers<8, FUSE>), grid, block, shm, s, a, wpb); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int BITS, bool FUSE>
+hipError_t launch_hamming_bits(const idb_pair_args &a, hipStream_t s)
+{
+    const uint32_t wpb = 4;
+    const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
+    if (a.HW <= 2) hipLaunchKernelGGL((k_pairs_hamming<2, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else if (a.HW <= 4) hipLaunchKernelGGL((k_pairs_hamming<4, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else if (a.HW <= 8) hipLaunchKernelGGL((k_pairs_hamming<8, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else if (a.HW <= 16) hipLaunchKernelGGL((k_pairs_hamming<16, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else if (a.HW <= 32) hipLaunchKernelGGL((k_pairs_hamming<32, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else if (a.HW <= 64) hipLaunchKernelGGL((k_pairs_hamming<64, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <bool FUSE>
+hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
+{
+    if (a.n_tiles == 0) return hipSuccess;
+    return a.bits == 2 ? launch_hamming_bits<2, FUSE>(a, s) : launch_hamming_bits<8, FUSE>(a, s);
+}
+
+}  // namespace
+
+hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<false>(a, W, s); }
+hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<true>(a, W, s); }
+hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<false>(a, s); }
+hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<true>(a, s); }

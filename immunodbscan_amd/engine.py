@@ -1,0 +1,122 @@
+"""Run helpers over the C ABI: one repertoire, a batch, or device-resident data."""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .configs import Clustering
+from .synth import Repertoire
+
+
+@dataclass
+class Result:
+    clone_id: np.ndarray
+    core: np.ndarray
+    copies: np.ndarray
+    uniques: np.ndarray
+    n_uniques: int | np.ndarray
+    counts: np.ndarray | None
+    stats: dict = field(default_factory=dict)
+
+
+_default_ctx: _lib.Context | None = None
+
+
+def default_context() -> _lib.Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = _lib.Context(0)
+    return _default_ctx
+
+
+def _params(cl: Clustering, tree_stats: bool, debug_hash_bits: int) -> _lib.IdbParams:
+    return _lib.make_params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, tree_stats, debug_hash_bits)
+
+
+def _dataset(rep: Repertoire) -> _lib.IdbDataset:
+    seq = rep.seq if len(rep.seq) else np.zeros(1, np.uint8)
+    return _lib.make_dataset(rep.n, seq, rep.off, rep.length, np.ascontiguousarray(rep.vbits), rep.vw,
+                             np.ascontiguousarray(rep.jbits), rep.jw, seq_bytes=len(rep.seq))
+
+
+def dbscan(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True,
+           tree_stats: bool = False, debug_hash_bits: int = 0) -> Result:
+    ctx = ctx or default_context()
+    n = rep.n
+    cid = np.zeros(n, np.uint32)
+    core = np.zeros(n, np.int32)
+    cop = np.zeros(n, np.uint32)
+    unq = np.zeros(n, np.uint32)
+    cnt = np.zeros(n, np.uint32) if counts else None
+    ds = _dataset(rep)
+    outs = _lib.make_outputs(cid, core, cop, unq, cnt)
+    st = _lib.IdbStats()
+    prm = _params(cl, tree_stats, debug_hash_bits)
+    ctx.check(_lib.lib.idb_dbscan(ctx.handle, ds, prm, outs, st))
+    return Result(cid, core, cop, unq, int(st.n_uniques), cnt, st.as_dict())
+
+
+def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True) -> Result:
+    ctx = ctx or default_context()
+    n = rep.n
+    starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
+    nr = len(starts) - 1
+    cid = np.zeros(n, np.uint32)
+    core = np.zeros(n, np.int32)
+    cop = np.zeros(n, np.uint32)
+    unq = np.zeros(n, np.uint32)
+    cnt = np.zeros(n, np.uint32) if counts else None
+    nu = np.zeros(max(nr, 1), np.uint32)
+    ds = _dataset(rep)
+    outs = _lib.make_outputs(cid, core, cop, unq, cnt)
+    st = _lib.IdbStats()
+    prm = _params(cl, False, 0)
+    ctx.check(_lib.lib.idb_dbscan_batch(ctx.handle, ds, starts.ctypes.data, nr, prm, outs, nu.ctypes.data, st))
+    return Result(cid, core, cop, unq, nu[:nr], cnt, st.as_dict())
+
+
+class DeviceRun:
+    """A repertoire resident in HBM (torch tensors used as device buffers) with
+    device-side outputs, for timing the path without host copies."""
+
+    def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, batch: bool = False):
+        import torch
+
+        self.ctx = ctx or default_context()
+        dev = torch.device("cuda", self.ctx.device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        self.rep = rep
+        self.seq = t(rep.seq if len(rep.seq) else np.zeros(1, np.uint8))
+        self.off = t(rep.off.view(np.int64))
+        self.len = t(rep.length.view(np.int32))
+        self.vb = t(rep.vbits)
+        self.jb = t(rep.jbits)
+        n = rep.n
+        z32 = lambda: torch.zeros(n, dtype=torch.int32, device=dev)  # noqa: E731
+        self.cid, self.core, self.cop, self.unq, self.cnt = z32(), z32(), z32(), z32(), z32()
+        self.ds = _lib.make_dataset(n, self.seq, self.off, self.len, self.vb, rep.vw, self.jb, rep.jw,
+                                    mem=_lib.IDB_MEM_DEVICE, seq_bytes=len(rep.seq))
+        self.outs = _lib.make_outputs(self.cid, self.core, self.cop, self.unq, self.cnt, mem=_lib.IDB_MEM_DEVICE)
+        self.prm = _params(cl, False, 0)
+        self.batch = batch
+        self.starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
+        self.nu = np.zeros(max(len(self.starts) - 1, 1), np.uint32)
+        self.stats = _lib.IdbStats()
+        torch.cuda.synchronize(dev)
+
+    def __call__(self) -> _lib.IdbStats:
+        if self.batch:
+            rc = _lib.lib.idb_dbscan_batch(self.ctx.handle, self.ds, self.starts.ctypes.data, len(self.starts) - 1,
+                                           self.prm, self.outs, self.nu.ctypes.data, self.stats)
+        else:
+            rc = _lib.lib.idb_dbscan(self.ctx.handle, self.ds, self.prm, self.outs, self.stats)
+        self.ctx.check(rc)
+        return self.stats
+
+    def result(self) -> Result:
+        g = lambda t: t.cpu().numpy()  # noqa: E731
+        return Result(g(self.cid).view(np.uint32), g(self.core), g(self.cop).view(np.uint32),
+                      g(self.unq).view(np.uint32), int(self.stats.n_uniques), g(self.cnt).view(np.uint32),
+                      self.stats.as_dict())

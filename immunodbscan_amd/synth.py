@@ -1,0 +1,160 @@
+"""Seeded synthetic repertoires (ctypes wrapper of libidb_synth.so)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+SYNTH_PATH = os.path.join(PKG, "libidb_synth.so")
+
+NT, AA = 0, 1
+LEN_UNIFORM, LEN_NORMAL = 0, 1
+SIZE_GEOMETRIC, SIZE_LOGNORMAL = 0, 1
+MUT_SUB, MUT_EDITS, MUT_SUB_INDEL = 0, 1, 2
+
+
+class SynthParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("n_rows", C.c_uint32),
+        ("n_seeds", C.c_uint32),
+        ("alphabet", C.c_uint32),
+        ("len_mode", C.c_uint32),
+        ("len_lo", C.c_uint32),
+        ("len_hi", C.c_uint32),
+        ("len_step", C.c_uint32),
+        ("len_mean", C.c_double),
+        ("len_sd", C.c_double),
+        ("size_mode", C.c_uint32),
+        ("size_p", C.c_double),
+        ("size_mu", C.c_double),
+        ("size_sigma", C.c_double),
+        ("mut_mode", C.c_uint32),
+        ("sub_rate", C.c_double),
+        ("indel_rate", C.c_double),
+        ("k_lo", C.c_uint32),
+        ("k_hi", C.c_uint32),
+        ("n_v", C.c_uint32),
+        ("n_j", C.c_uint32),
+        ("v_zipf_s", C.c_double),
+        ("j_zipf_s", C.c_double),
+        ("v_ambig", C.c_double),
+        ("j_ambig", C.c_double),
+        ("shuffle", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise ImportError(f"{SYNTH_PATH} is not built; run `python -m immunodbscan_amd.build`")
+        lib = C.CDLL(SYNTH_PATH)
+        lib.idb_synth_default.argtypes = [C.POINTER(SynthParams)]
+        lib.idb_synth_generate.argtypes = [C.POINTER(SynthParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]
+        lib.idb_synth_generate.restype = C.c_int64
+        lib.idb_synth_lognormal_sizes.argtypes = [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                                                  C.c_uint32, C.c_void_p]
+        lib.idb_synth_lognormal_sizes.restype = C.c_int32
+        _lib = lib
+    return _lib
+
+
+@dataclass
+class Repertoire:
+    """Rows of one (or several concatenated) repertoires in the C-ABI layout."""
+
+    seq: np.ndarray          # uint8, concatenated
+    off: np.ndarray          # uint64
+    length: np.ndarray       # uint32
+    vbits: np.ndarray        # int32 [n, vw]
+    jbits: np.ndarray        # int32 [n, jw]
+    clone_of: np.ndarray     # uint32 ground-truth clonotype
+    rep_start: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+
+    @property
+    def n(self) -> int:
+        return int(self.length.shape[0])
+
+    @property
+    def vw(self) -> int:
+        return int(self.vbits.shape[1])
+
+    @property
+    def jw(self) -> int:
+        return int(self.jbits.shape[1])
+
+    def row(self, i: int) -> bytes:
+        o = int(self.off[i])
+        return self.seq[o:o + int(self.length[i])].tobytes()
+
+    def subset(self, rows: np.ndarray) -> "Repertoire":
+        rows = np.asarray(rows, dtype=np.int64)
+        lens = self.length[rows]
+        off = np.zeros(len(rows), np.uint64)
+        if len(rows):
+            off[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+        parts = [self.seq[int(self.off[r]):int(self.off[r]) + int(self.length[r])] for r in rows]
+        seq = np.concatenate(parts) if parts else np.zeros(0, np.uint8)
+        return Repertoire(seq.astype(np.uint8), off, lens.astype(np.uint32), self.vbits[rows].copy(),
+                          self.jbits[rows].copy(), self.clone_of[rows].copy())
+
+
+def default_params() -> SynthParams:
+    p = SynthParams()
+    _load().idb_synth_default(C.byref(p))
+    return p
+
+
+def generate(p: SynthParams, vw: int | None = None, jw: int | None = None) -> Repertoire:
+    lib = _load()
+    vw = vw or max(1, (p.n_v + 31) // 32)
+    jw = jw or max(1, (p.n_j + 31) // 32)
+    n = p.n_rows
+    off = np.zeros(n, np.uint64)
+    length = np.zeros(n, np.uint32)
+    vb = np.zeros((n, vw), np.int32)
+    jb = np.zeros((n, jw), np.int32)
+    clone = np.zeros(n, np.uint32)
+    cap = n * (p.len_hi + 16) + 64
+    seq = np.zeros(cap, np.uint8)
+    need = lib.idb_synth_generate(C.byref(p), seq.ctypes.data, cap, off.ctypes.data, length.ctypes.data,
+                                  vb.ctypes.data, vw, jb.ctypes.data, jw, clone.ctypes.data)
+    if need < 0:
+        raise ValueError("invalid synthetic repertoire parameters")
+    if need > cap:
+        seq = np.zeros(need, np.uint8)
+        need = lib.idb_synth_generate(C.byref(p), seq.ctypes.data, need, off.ctypes.data, length.ctypes.data,
+                                      vb.ctypes.data, vw, jb.ctypes.data, jw, clone.ctypes.data)
+    return Repertoire(seq[:need].copy(), off, length, vb, jb, clone)
+
+
+def lognormal_sizes(seed: int, n: int, median: float, sigma: float, lo: int, hi: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    _load().idb_synth_lognormal_sizes(seed, n, median, sigma, lo, hi, out.ctypes.data)
+    return out
+
+
+def concat(reps: list[Repertoire]) -> Repertoire:
+    """Back-to-back rows of several repertoires, with rep_start offsets."""
+    n = [r.n for r in reps]
+    start = np.zeros(len(reps) + 1, np.uint32)
+    start[1:] = np.cumsum(n)
+    seq_sizes = [len(r.seq) for r in reps]
+    seq_base = np.zeros(len(reps), np.uint64)
+    seq_base[1:] = np.cumsum(seq_sizes[:-1], dtype=np.uint64)
+    off = np.concatenate([r.off + seq_base[k] for k, r in enumerate(reps)]).astype(np.uint64)
+    out = Repertoire(np.concatenate([r.seq for r in reps]), off,
+                     np.concatenate([r.length for r in reps]),
+                     np.concatenate([r.vbits for r in reps]),
+                     np.concatenate([r.jbits for r in reps]),
+                     np.concatenate([r.clone_of for r in reps]))
+    out.rep_start = start
+    return out

@@ -1,0 +1,133 @@
+/*
+ * idb.h — C ABI of libidb_hip.so, the MI355X (gfx950) implementation of
+ * immunodbscan's DBSCAN clone-collapsing path.
+ *
+ * The reference crosses from host C++ into its device code through one class,
+ * InterlinkClass (cuda/Interlink.hpp:46-69, cuda/Interlink.cu:34-95).  Its
+ * constructor turns gene calls into bitsets, packs the sequences, runs
+ * DBSCAN::doScan (cuda/PSDBSCAN.hcu:1300-1356) and copies clone ids, core
+ * flags, uniques and copies back.  The entry points below are what a binding
+ * of that class needs: the gene-bitset builder (bitset_container,
+ * cuda/BitsetVector.inl:27-45) and the clustering run.  A drop-in C++
+ * InterlinkClass built on them is in immunodbscan_amd/csrc/Interlink.{hpp,cpp}.
+ *
+ * Conventions: every function returns IDB_OK (0) or a negative IDB_ERR_*
+ * code; idb_last_error() then describes the failure.  Calls are synchronous.
+ * One context per host thread.  Output buffers are owned by the caller.
+ */
+#ifndef IDB_H
+#define IDB_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IDB_OK 0
+#define IDB_ERR_ARG (-1)          /* bad argument or inconsistent sizes          */
+#define IDB_ERR_HIP (-2)          /* a HIP runtime call failed                    */
+#define IDB_ERR_OOM (-3)          /* device allocation failed or cap exceeded     */
+#define IDB_ERR_UNSUPPORTED (-4)  /* input outside the supported envelope         */
+
+#define IDB_MEM_HOST 0            /* arrays are host memory                       */
+#define IDB_MEM_DEVICE 1          /* arrays are device memory on the ctx device   */
+
+#define IDB_N_STAGES 8
+
+typedef struct idb_ctx idb_ctx;
+
+/* Rows after the reference's empty-cell removal (cpp/TableHolder.hpp:283-285),
+ * in table order.  Row i's sequence is seq[off[i] .. off[i]+len[i]); its V and
+ * J gene bitsets are v_width / j_width int32 words, gene g in word g>>5 at bit
+ * g&31 (cuda/BitsetVector.inl:34,73-82). */
+typedef struct {
+    uint32_t n_rows;
+    uint32_t mem;                 /* IDB_MEM_HOST or IDB_MEM_DEVICE */
+    const uint8_t *seq;
+    uint64_t seq_bytes;
+    const uint64_t *off;
+    const uint32_t *len;
+    const int32_t *vbits;
+    uint32_t v_width;
+    const int32_t *jbits;
+    uint32_t j_width;
+} idb_dataset;
+
+/* The four hot-path settings of cpp/ConfigHolder.hpp:181-199, plus an absolute
+ * threshold mode (t = abs_k edits for every pair) that the reference lacks. */
+typedef struct {
+    float eps;                    /* maximum_relative_edit_distance; t = trunc(max(lx,ly)*eps) */
+    int32_t abs_k;                /* >= 0 selects the absolute mode; -1 = relative             */
+    uint32_t min_pts;             /* minimum_number_of_neighbors                               */
+    uint32_t use_hamming;         /* use_hamming                                               */
+    uint32_t tree_stats;          /* get_tree_depth_statistics                                 */
+    uint32_t debug_hash_bits;     /* 0 = full 64-bit duplicate hash; 1..63 keeps only that
+                                     many bits (forces collisions in tests)                   */
+    uint32_t reserved[2];
+} idb_params;
+
+/* Per-row outputs (cuda/Interlink.hpp:48-49 and :59-67).  uniques holds the
+ * ascending representative indices in its first n_uniques entries and 0 after.
+ * neighbour_count (optional, may be NULL) receives the exact copies-weighted
+ * neighbour count of every representative and 0 for duplicate rows. */
+typedef struct {
+    uint32_t mem;                 /* IDB_MEM_HOST or IDB_MEM_DEVICE */
+    uint32_t *clone_id;
+    int32_t *core;
+    uint32_t *copies;
+    uint32_t *uniques;
+    uint32_t *neighbour_count;
+} idb_outputs;
+
+/* Counters the reference writes to its log (cuda/PSDBSCAN.hcu:1113,1198,1279,1290)
+ * plus work counters and stage times.  Stage order matches the reference's
+ * timer names: duplicates, duplicate-tree reduction, copies, core points,
+ * grow clusters, cluster-tree reduction, renumber, whole run. */
+typedef struct {
+    uint32_t n_rows, n_uniques, n_cores, n_clusters, n_noise;
+    uint32_t max_len, alphabet_size, n_buckets;
+    uint64_t bucket_entries, tiles;
+    uint64_t gated_pairs;         /* pairs reaching a distance kernel (count pass)   */
+    uint64_t neighbour_pairs;     /* unordered unique pairs with N(x,y)              */
+    uint64_t fuse_pairs;          /* pairs evaluated by the fuse pass                */
+    uint64_t gated_letters;       /* text letters consumed by the count pass's gated pairs */
+    float count_kernel_ms;        /* count-pass kernel, HIP events on the library stream */
+    float fuse_kernel_ms;         /* fuse-pass kernel                                */
+    uint32_t tree_depth[51];      /* union-find depth histogram when tree_stats      */
+    float stage_ms[IDB_N_STAGES];
+} idb_stats;
+
+/* Context on HIP device `device`: a stream and a reusable device arena.
+ * Replaces the implicit device-0 setup of DBSCAN (cuda/PSDBSCAN.hcu:962-990). */
+int idb_create(idb_ctx **ctx, int device);
+void idb_destroy(idb_ctx *ctx);
+const char *idb_last_error(const idb_ctx *ctx);
+
+/* R2 gene bitsets on the host: bit g of row i is set iff gene name g occurs as
+ * a substring of call string i.  Replaces bitset_container's constructor and
+ * its per-gene find_string_occurences launches (cuda/BitsetVector.inl:27-45,
+ * 118-209).  Strings are given as concatenated bytes with offsets/lengths;
+ * out has n * width words, width = ceil(n_genes / 32). */
+int idb_gene_bitsets(uint32_t n, const uint8_t *calls, const uint64_t *call_off, const uint32_t *call_len,
+                     uint32_t n_genes, const uint8_t *genes, const uint64_t *gene_off, const uint32_t *gene_len,
+                     int32_t *out, uint32_t width);
+
+/* One repertoire: duplicates, copies, core points, clusters and renumbering
+ * (DBSCAN::doScan, cuda/PSDBSCAN.hcu:1300-1356, as driven by
+ * InterlinkClass::InterlinkClass, cuda/Interlink.cu:34-95).  stats may be NULL. */
+int idb_dbscan(idb_ctx *ctx, const idb_dataset *data, const idb_params *params, idb_outputs *out,
+               idb_stats *stats);
+
+/* Many repertoires in one pass: data holds the rows of all repertoires back to
+ * back, repertoire r being rows rep_start[r] .. rep_start[r+1]-1.  Results
+ * equal n_reps separate idb_dbscan calls: clone ids restart at 1, and uniques
+ * holds repertoire-local indices at each repertoire's row offset, with the
+ * per-repertoire count in n_uniques[r].  No reference counterpart: the
+ * reference runs one table per process (cpp/psdbscan.cpp:25-106). */
+int idb_dbscan_batch(idb_ctx *ctx, const idb_dataset *data, const uint32_t *rep_start, uint32_t n_reps,
+                     const idb_params *params, idb_outputs *out, uint32_t *n_uniques, idb_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
