@@ -67,7 +67,8 @@ class IdbParams(C.Structure):
         ("use_hamming", C.c_uint32),
         ("tree_stats", C.c_uint32),
         ("debug_hash_bits", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("edge_capacity", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -92,6 +93,8 @@ class IdbStats(C.Structure):
         ("max_len", C.c_uint32),
         ("alphabet_size", C.c_uint32),
         ("n_buckets", C.c_uint32),
+        ("fuse_from_edges", C.c_uint32),
+        ("reserved_", C.c_uint32),
         ("bucket_entries", C.c_uint64),
         ("tiles", C.c_uint64),
         ("gated_pairs", C.c_uint64),
@@ -192,7 +195,7 @@ class Context:
 
 
 def make_params(eps: float = 0.15, abs_k: int = -1, min_pts: int = 2, use_hamming: bool = False,
-                tree_stats: bool = False, debug_hash_bits: int = 0) -> IdbParams:
+                tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0) -> IdbParams:
     p = IdbParams()
     p.eps = float(np.float32(eps))
     p.abs_k = int(abs_k)
@@ -200,6 +203,7 @@ def make_params(eps: float = 0.15, abs_k: int = -1, min_pts: int = 2, use_hammin
     p.use_hamming = 1 if use_hamming else 0
     p.tree_stats = 1 if tree_stats else 0
     p.debug_hash_bits = int(debug_hash_bits)
+    p.edge_capacity = int(edge_capacity)
     return p
 
 

@@ -39,7 +39,7 @@ enum Slot {
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
-    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART,
+    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES,
     S_COUNT
 };
 
@@ -806,8 +806,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(hilen, hl.data(), sizeof(uint16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
     check(c, hipMemcpyAsync(cmap, cmap_h, 256, hipMemcpyHostToDevice, s), "cmap");
 
-    unsigned long long *work = buf<unsigned long long>(c, S_WORK, 4);
-    check(c, hipMemsetAsync(work, 0, 4 * sizeof(unsigned long long), s), "memset");
+    unsigned long long *work = buf<unsigned long long>(c, S_WORK, 8);
+    check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "memset");
+    unsigned long long edge_cap = 0;
     uint32_t B = 0, T = 0;
     idb_pair_args a{};
     if (E > 0) {
@@ -864,8 +865,16 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.gvj = gvj;
         a.single_bucket = single;
         a.tlut = tlut_c;
+        a.tlut_f = tlut_f;
         a.cnt = cnt;
         a.work = work;
+        // Neighbour pairs found by the count pass are kept for the fuse step;
+        // if they overflow the list, the fuse pass re-evaluates the tiles.
+        edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 20, 8ull * E);
+        edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 29);
+        a.edges = buf<uint2>(c, S_EDGES, edge_cap);
+        a.edge_cap = edge_cap;
+        a.n_edges = work + 4;
         check(c, hipEventRecord(c->kev[0], s), "event record");
         check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
         check(c, hipEventRecord(c->kev[1], s), "event record");
@@ -874,15 +883,25 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, small);
     record(c, 3);
 
-    // --- Grow clusters
+    // --- Grow clusters: union over the recorded neighbour pairs, or a fuse
+    // pass over the tiles when the pair list overflowed
+    unsigned long long n_edges = 0;
+    bool from_edges = false;
     if (E > 0) {
-        a.tlut = tlut_f;
-        a.core = core;
-        a.par = par;
-        a.bmin = bmin;
-        a.cnt = nullptr;
+        readback(c, &n_edges, work + 4, sizeof(n_edges));
+        from_edges = n_edges <= edge_cap;
         check(c, hipEventRecord(c->kev[2], s), "event record");
-        check(c, ham ? idb_launch_fuse_hamming(a, s) : idb_launch_fuse_myers(a, W, s), "fuse kernel launch");
+        if (from_edges) {
+            check(c, idb_launch_union_edges(a.edges, n_edges, core, par, bmin, s), "union kernel launch");
+        } else {
+            a.tlut = tlut_f;
+            a.core = core;
+            a.par = par;
+            a.bmin = bmin;
+            a.cnt = nullptr;
+            a.edges = nullptr;
+            check(c, ham ? idb_launch_fuse_hamming(a, s) : idb_launch_fuse_myers(a, W, s), "fuse kernel launch");
+        }
         check(c, hipEventRecord(c->kev[3], s), "event record");
     }
     record(c, 4);
@@ -933,7 +952,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         check(c, hipMemcpyAsync(us.data(), ustart, sizeof(uint32_t) * (n_reps + 1), hipMemcpyDeviceToHost, s), "ustart");
     }
     uint32_t fin[128];
-    unsigned long long wk[4];
+    unsigned long long wk[8];
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
@@ -952,6 +971,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->neighbour_pairs = wk[1];
         st->fuse_pairs = wk[2];
         st->gated_letters = wk[3];
+        st->fuse_from_edges = from_edges ? 1u : 0u;
         if (E > 0 && T > 0) {
             (void)hipEventElapsedTime(&st->count_kernel_ms, c->kev[0], c->kev[1]);
             (void)hipEventElapsedTime(&st->fuse_kernel_ms, c->kev[2], c->kev[3]);

@@ -49,11 +49,15 @@ struct idb_pair_args {
     uint32_t gvj;                 // genes pairs per repertoire (32*vw*gj)
     uint32_t single_bucket;       // 1: every pair is gated explicitly
     const int16_t *tlut;          // threshold by max length (count or fuse pass)
+    const int16_t *tlut_f;        // fuse-pass threshold (edges recorded by the count pass)
     int32_t *cnt;                 // count pass output
     const int32_t *core;          // fuse pass input (per row)
     uint32_t *par;                // fuse pass: union-find parents
     uint32_t *bmin;               // fuse pass: border claims
     unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] text letters
+    uint2 *edges;                 // count pass: neighbour pairs (x, y), may be null
+    unsigned long long edge_cap;
+    unsigned long long *n_edges;
 };
 
 // Kernel launchers (defined in idb_pairs.hip).
@@ -61,3 +65,5 @@ hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s);
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s);
+hipError_t idb_launch_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
+                                  uint32_t *bmin, hipStream_t s);

@@ -101,6 +101,23 @@ __device__ bool pair_in_bucket(const idb_pair_args &a, uint32_t x, uint32_t y, u
     return lv == (int)(bl / a.gj) && lj == (int)(bl % a.gj);
 }
 
+// Append the wave's neighbour pairs to the edge list.  Past the capacity the
+// 64-bit counter keeps growing, so the host sees the overflow and runs the
+// fuse pass over the tiles instead.
+__device__ __forceinline__ void emit_edges(const idb_pair_args &a, bool keep, uint32_t x, uint32_t y, uint32_t lane)
+{
+    const unsigned long long km = __ballot(keep);
+    if (!km || !a.edges) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(a.n_edges, (unsigned long long)__popcll(km));
+    base = __shfl(base, 0, 64);
+    if (keep) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
+        const unsigned long long r = base + below;
+        if (r < a.edge_cap) a.edges[r] = make_uint2(x, y);
+    }
+}
+
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
 // walks a contiguous run of tiles (neighbouring tiles share query blocks,
 // which then stay in that XCD's L2).
@@ -200,6 +217,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 if (hit) acc_y += (int32_t)a.e_cp[q];
                 const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
                 if (lane == 0) atomicAdd(a.cnt + x, s);
+                emit_edges(a, hit, x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
                 else if (core_x) atomicMin(a.bmin + y, x);
@@ -284,6 +302,10 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
                 if (hit) acc_y += (int32_t)a.e_cp[q];
                 const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
                 if (lane == 0) atomicAdd(a.cnt + x, s);
+                // the fuse pass compares after every letter (PSDBSCAN.hcu:709-712):
+                // with a negative threshold no non-empty pair is an edge there
+                const int32_t tf = (int32_t)a.tlut_f[lx];
+                emit_edges(a, hit && ((int32_t)ham <= tf || lx == 0), x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
                 else if (core_x) atomicMin(a.bmin + y, x);
@@ -295,6 +317,19 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
         if (!FUSE) atomicAdd(a.work + 1, hits);
+    }
+}
+
+__global__ void k_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
+                              uint32_t *bmin)
+{
+    for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
+         e += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint2 p = edges[e];
+        const int32_t cx = core[p.x], cy = core[p.y];
+        if (cx && cy) uf_union(par, p.x, p.y);
+        else if (cx) atomicMin(bmin + p.y, p.x);
+        else if (cy) atomicMin(bmin + p.x, p.y);
     }
 }
 
@@ -360,3 +395,13 @@ hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<true>(a, W, s); }
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<false>(a, s); }
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<true>(a, s); }
+
+hipError_t idb_launch_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
+                                  uint32_t *bmin, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    unsigned long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_union_edges, dim3((uint32_t)blocks), dim3(256), 0, s, edges, n, core, par, bmin);
+    return hipGetLastError();
+}

@@ -31,8 +31,9 @@ def default_context() -> _lib.Context:
     return _default_ctx
 
 
-def _params(cl: Clustering, tree_stats: bool, debug_hash_bits: int) -> _lib.IdbParams:
-    return _lib.make_params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, tree_stats, debug_hash_bits)
+def _params(cl: Clustering, tree_stats: bool, debug_hash_bits: int, edge_capacity: int = 0) -> _lib.IdbParams:
+    return _lib.make_params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, tree_stats, debug_hash_bits,
+                            edge_capacity)
 
 
 def _dataset(rep: Repertoire) -> _lib.IdbDataset:
@@ -42,7 +43,7 @@ def _dataset(rep: Repertoire) -> _lib.IdbDataset:
 
 
 def dbscan(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True,
-           tree_stats: bool = False, debug_hash_bits: int = 0) -> Result:
+           tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0) -> Result:
     ctx = ctx or default_context()
     n = rep.n
     cid = np.zeros(n, np.uint32)
@@ -53,12 +54,13 @@ def dbscan(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, cou
     ds = _dataset(rep)
     outs = _lib.make_outputs(cid, core, cop, unq, cnt)
     st = _lib.IdbStats()
-    prm = _params(cl, tree_stats, debug_hash_bits)
+    prm = _params(cl, tree_stats, debug_hash_bits, edge_capacity)
     ctx.check(_lib.lib.idb_dbscan(ctx.handle, ds, prm, outs, st))
     return Result(cid, core, cop, unq, int(st.n_uniques), cnt, st.as_dict())
 
 
-def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True) -> Result:
+def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True,
+                 edge_capacity: int = 0) -> Result:
     ctx = ctx or default_context()
     n = rep.n
     starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
@@ -72,7 +74,7 @@ def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = Non
     ds = _dataset(rep)
     outs = _lib.make_outputs(cid, core, cop, unq, cnt)
     st = _lib.IdbStats()
-    prm = _params(cl, False, 0)
+    prm = _params(cl, False, 0, edge_capacity)
     ctx.check(_lib.lib.idb_dbscan_batch(ctx.handle, ds, starts.ctypes.data, nr, prm, outs, nu.ctypes.data, st))
     return Result(cid, core, cop, unq, nu[:nr], cnt, st.as_dict())
 

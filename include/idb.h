@@ -63,7 +63,9 @@ typedef struct {
     uint32_t tree_stats;          /* get_tree_depth_statistics                                 */
     uint32_t debug_hash_bits;     /* 0 = full 64-bit duplicate hash; 1..63 keeps only that
                                      many bits (forces collisions in tests)                   */
-    uint32_t reserved[2];
+    uint32_t edge_capacity;       /* 0 = automatic; else the most neighbour pairs kept from the
+                                     count pass (more fall back to a second pass over pairs)  */
+    uint32_t reserved;
 } idb_params;
 
 /* Per-row outputs (cuda/Interlink.hpp:48-49 and :59-67).  uniques holds the
@@ -86,6 +88,8 @@ typedef struct {
 typedef struct {
     uint32_t n_rows, n_uniques, n_cores, n_clusters, n_noise;
     uint32_t max_len, alphabet_size, n_buckets;
+    uint32_t fuse_from_edges;     /* 1: clusters grown from the count pass's pair list       */
+    uint32_t reserved_;
     uint64_t bucket_entries, tiles;
     uint64_t gated_pairs;         /* pairs reaching a distance kernel (count pass)   */
     uint64_t neighbour_pairs;     /* unordered unique pairs with N(x,y)              */
