@@ -77,10 +77,16 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    import numpy as np
+    # torch's wheel carries its own HIP runtime: initialise it before the
+    # library's runtime so that both come up, then use it only to bracket the
+    # timed region.
     import torch
 
-    from immunodbscan_amd import configs, engine, synth
+    torch.cuda.init()
+    torch.cuda.synchronize()
+    import numpy as np
+
+    from immunodbscan_amd import configs, engine, hip, synth
 
     cl = configs.C2
     rep = synth.generate(configs.c2_params(n_rows=args.rows, n_seeds=max(1, args.rows // 10)))
@@ -89,6 +95,7 @@ def main() -> int:
     for w in range(args.warmup):
         run()
         log(f"warmup {w + 1}/{args.warmup} done")
+    hip.synchronize()
     torch.cuda.synchronize()
     cnt_ms, fuse_ms, st = [], [], None
     t0 = time.perf_counter()
@@ -96,6 +103,7 @@ def main() -> int:
         st = run()
         cnt_ms.append(st.count_kernel_ms)
         fuse_ms.append(st.fuse_kernel_ms)
+    hip.synchronize()
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     ms = T / args.steps * 1e3

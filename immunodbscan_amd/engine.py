@@ -80,23 +80,23 @@ def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = Non
 
 
 class DeviceRun:
-    """A repertoire resident in HBM (torch tensors used as device buffers) with
-    device-side outputs, for timing the path without host copies."""
+    """A repertoire resident in HBM with device-side outputs, for timing the
+    path without host copies.  Buffers come from immunodbscan_amd.hip."""
 
     def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, batch: bool = False):
-        import torch
+        from .hip import DeviceBuffer, synchronize
 
         self.ctx = ctx or default_context()
-        dev = torch.device("cuda", self.ctx.device)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        dev = self.ctx.device
+        up = lambda a: DeviceBuffer.from_array(a, dev)  # noqa: E731
         self.rep = rep
-        self.seq = t(rep.seq if len(rep.seq) else np.zeros(1, np.uint8))
-        self.off = t(rep.off.view(np.int64))
-        self.len = t(rep.length.view(np.int32))
-        self.vb = t(rep.vbits)
-        self.jb = t(rep.jbits)
+        self.seq = up(rep.seq if len(rep.seq) else np.zeros(1, np.uint8))
+        self.off = up(rep.off)
+        self.len = up(rep.length)
+        self.vb = up(rep.vbits)
+        self.jb = up(rep.jbits)
         n = rep.n
-        z32 = lambda: torch.zeros(n, dtype=torch.int32, device=dev)  # noqa: E731
+        z32 = lambda: DeviceBuffer.zeros(4 * n, dev)  # noqa: E731
         self.cid, self.core, self.cop, self.unq, self.cnt = z32(), z32(), z32(), z32(), z32()
         self.ds = _lib.make_dataset(n, self.seq, self.off, self.len, self.vb, rep.vw, self.jb, rep.jw,
                                     mem=_lib.IDB_MEM_DEVICE, seq_bytes=len(rep.seq))
@@ -106,7 +106,7 @@ class DeviceRun:
         self.starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
         self.nu = np.zeros(max(len(self.starts) - 1, 1), np.uint32)
         self.stats = _lib.IdbStats()
-        torch.cuda.synchronize(dev)
+        synchronize()
 
     def __call__(self) -> _lib.IdbStats:
         if self.batch:
@@ -118,7 +118,8 @@ class DeviceRun:
         return self.stats
 
     def result(self) -> Result:
-        g = lambda t: t.cpu().numpy()  # noqa: E731
-        return Result(g(self.cid).view(np.uint32), g(self.core), g(self.cop).view(np.uint32),
-                      g(self.unq).view(np.uint32), int(self.stats.n_uniques), g(self.cnt).view(np.uint32),
-                      self.stats.as_dict())
+        n = self.rep.n
+        return Result(self.cid.to_numpy(np.uint32, n), self.core.to_numpy(np.int32, n),
+                      self.cop.to_numpy(np.uint32, n), self.unq.to_numpy(np.uint32, n),
+                      int(self.stats.n_uniques) if not self.batch else self.nu.copy(),
+                      self.cnt.to_numpy(np.uint32, n), self.stats.as_dict())

@@ -254,3 +254,41 @@ def test_repeat_runs_are_identical(gpu_ctx):
             assert np.array_equal(getattr(a, k), getattr(b, k)), k
         for k in ("gated_pairs", "neighbour_pairs", "tiles", "bucket_entries"):
             assert a.stats[k] == b.stats[k], k
+
+
+def test_interlink_cpp_drop_in(gpu_ctx, tmp_path):
+    """The C++ InterlinkClass (same declarations as the reference's
+    cuda/Interlink.hpp) driven like TableHolder::do_the_clustering."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "interlink_demo")
+    assert os.path.exists(exe), "interlink_demo not built"
+    rng = random.Random(14)
+    vh = [ln.decode() for ln in open("tests/golden/VHgenes.txt", "rb").read().split(b"\n") if ln]
+    jh = [ln.decode() for ln in open("tests/golden/JHgenes.txt", "rb").read().split(b"\n") if ln]
+    rows = random_rows(rng, 2000, b"ACGT", 30, 60, len(vh), len(jh), 0.2, families=120, edits=2)
+    tsv = tmp_path / "rows.tsv"
+    with open(tsv, "w") as f:
+        for s, v, j in rows:
+            f.write(", or ".join(f"Homsap {vh[g][:-1]}*01 F" for g in v) + "\t"
+                    + ", or ".join(f"Homsap {jh[g][:-1]}*01 F" for g in j) + "\t" + s.decode() + "\n")
+    for eps, minpts, ham in ((0.15, 2, 1), (0.1, 3, 0)):
+        out = subprocess.run([exe, str(tsv), "tests/golden/VHgenes.txt", "tests/golden/JHgenes.txt", str(eps),
+                              str(minpts), str(ham)], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        lines = [ln for ln in out.stdout.split("\n") if ln and not ln.startswith("#")]
+        got = np.array([[int(x) for x in ln.split("\t")] for ln in lines])
+        vb = O.gene_bitsets([(", or ".join(f"Homsap {vh[g][:-1]}*01 F" for g in v)).encode() for _, v, _ in rows],
+                            [g.encode() for g in vh])
+        jb = O.gene_bitsets([(", or ".join(f"Homsap {jh[g][:-1]}*01 F" for g in j)).encode() for _, _, j in rows],
+                            [g.encode() for g in jh])
+        seqs = [s for s, _, _ in rows]
+        lens = np.array([len(s) for s in seqs], np.uint32)
+        off = np.concatenate([[0], np.cumsum(lens[:-1])]).astype(np.uint64)
+        exp = O.dbscan(O.Data(np.frombuffer(b"".join(seqs), np.uint8), off, lens, vb, jb),
+                       O.params(eps, -1, minpts, bool(ham)))
+        assert np.array_equal(got[:, 0], exp["clone_id"])
+        assert np.array_equal(got[:, 1], exp["copies"])
+        assert np.array_equal(got[:, 2], exp["core"])
+        assert f"# n_uniques {exp['n_uniques']}" in out.stdout
