@@ -68,9 +68,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=200_000)
-    ap.add_argument("--cpu-rows", type=int, default=100_000)
+    ap.add_argument("--cpu-rows", type=int, default=200_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the timed steps, run one device-to-device copy of this many MiB "
+                         "(the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -91,6 +94,14 @@ def main() -> int:
     cl = configs.C2
     rep = synth.generate(configs.c2_params(n_rows=args.rows, n_seeds=max(1, args.rows // 10)))
     log(f"generated {rep.n} rows, {len(rep.seq)} letters")
+    if args.copy_calibration_mib:
+        nb = args.copy_calibration_mib << 20
+        src, dst = hip.DeviceBuffer.zeros(nb), hip.DeviceBuffer(nb)
+        hip.copy_device(dst, src, nb)
+        hip.synchronize()
+        src.free()
+        dst.free()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB device-to-device")
     run = engine.DeviceRun(rep, cl)
     for w in range(args.warmup):
         run()

@@ -718,7 +718,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     } else {
         W = std::max(1u, (maxlen + 31) / 32);
         if (W > IDB_MAX_W) fail(c, IDB_ERR_UNSUPPORTED, "Levenshtein rows longer than 256 letters are not supported");
-        if ((size_t)K * W * 64 * 4 > 65536)
+        if (idb_myers_lds_per_wave(std::max(K, 1u), W) > idb_lds_limit())
             fail(c, IDB_ERR_UNSUPPORTED, "alphabet of " + std::to_string(K) + " letters with rows of " +
                                              std::to_string(maxlen) + " letters exceeds the LDS budget");
         S = std::max(4u, (maxlen + 3) / 4 * 4);

@@ -60,6 +60,10 @@ struct idb_pair_args {
     unsigned long long *n_edges;
 };
 
+// LDS bytes one wave of the Myers kernels needs, and the device's limit per block.
+size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W);
+size_t idb_lds_limit();
+
 // Kernel launchers (defined in idb_pairs.hip).
 hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);

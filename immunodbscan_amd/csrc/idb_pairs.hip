@@ -24,6 +24,8 @@
 // component minima), and a non-core row records its minimum-index core
 // neighbour with atomicMin (the canonical form of the reference's first-come
 // atomicCAS at :113-122).
+#include <algorithm>
+
 #include "idb_internal.h"
 #include "pairdist.h"
 
@@ -101,21 +103,42 @@ __device__ bool pair_in_bucket(const idb_pair_args &a, uint32_t x, uint32_t y, u
     return lv == (int)(bl / a.gj) && lj == (int)(bl % a.gj);
 }
 
-// Append the wave's neighbour pairs to the edge list.  Past the capacity the
-// 64-bit counter keeps growing, so the host sees the overflow and runs the
-// fuse pass over the tiles instead.
-__device__ __forceinline__ void emit_edges(const idb_pair_args &a, bool keep, uint32_t x, uint32_t y, uint32_t lane)
+// Neighbour pairs are staged per wave in LDS (IDB_TILE entries) and appended
+// to the global edge list one chunk at a time, so the list's counter sees one
+// atomic per chunk instead of one per hit.  Past the capacity the 64-bit
+// counter keeps growing, so the host sees the overflow and runs the fuse pass
+// over the tiles instead.
+struct EdgeStage {
+    uint2 *buf;   // this wave's IDB_TILE staging slots
+    uint32_t n;   // wave-uniform fill level
+};
+
+__device__ __forceinline__ void flush_edges(const idb_pair_args &a, EdgeStage &st, uint32_t lane)
+{
+    if (st.n == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(a.n_edges, (unsigned long long)st.n);
+    base = __shfl(base, 0, 64);
+    if (lane < st.n) {
+        const unsigned long long r = base + lane;
+        if (r < a.edge_cap) a.edges[r] = st.buf[lane];
+    }
+    st.n = 0;
+}
+
+__device__ __forceinline__ void stage_edges(const idb_pair_args &a, EdgeStage &st, bool keep, uint32_t x, uint32_t y,
+                                            uint32_t lane)
 {
     const unsigned long long km = __ballot(keep);
     if (!km || !a.edges) return;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(a.n_edges, (unsigned long long)__popcll(km));
-    base = __shfl(base, 0, 64);
+    const uint32_t cnt = (uint32_t)__popcll(km);
+    if (st.n + cnt > IDB_TILE) flush_edges(a, st, lane);
     if (keep) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0u));
-        const unsigned long long r = base + below;
-        if (r < a.edge_cap) a.edges[r] = make_uint2(x, y);
+        st.buf[st.n + below] = make_uint2(x, y);
     }
+    st.n += cnt;
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
@@ -134,7 +157,8 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t K = a.K;
-    uint32_t *peq = lds + (size_t)wib * K * W * 64;
+    uint32_t *peq = lds + (size_t)wib * (K * W * 64 + 2 * IDB_TILE);
+    EdgeStage est{(uint2 *)(peq + K * W * 64), 0u};
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
     unsigned long long gated = 0, hits = 0, letters = 0;
@@ -217,7 +241,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 if (hit) acc_y += (int32_t)a.e_cp[q];
                 const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
                 if (lane == 0) atomicAdd(a.cnt + x, s);
-                emit_edges(a, hit, x, y, lane);
+                stage_edges(a, est, hit, x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
                 else if (core_x) atomicMin(a.bmin + y, x);
@@ -226,6 +250,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
     }
+    if (!FUSE) flush_edges(a, est, lane);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
@@ -243,6 +268,8 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
     const uint32_t HW = a.HW;
+    __shared__ uint2 stage[4][IDB_TILE];
+    EdgeStage est{stage[wib], 0u};
     unsigned long long gated = 0, hits = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
@@ -305,7 +332,7 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
                 // the fuse pass compares after every letter (PSDBSCAN.hcu:709-712):
                 // with a negative threshold no non-empty pair is an edge there
                 const int32_t tf = (int32_t)a.tlut_f[lx];
-                emit_edges(a, hit && ((int32_t)ham <= tf || lx == 0), x, y, lane);
+                stage_edges(a, est, hit && ((int32_t)ham <= tf || lx == 0), x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
                 else if (core_x) atomicMin(a.bmin + y, x);
@@ -314,6 +341,7 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
         }
         if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
     }
+    if (!FUSE) flush_edges(a, est, lane);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
         if (!FUSE) atomicAdd(a.work + 1, hits);
@@ -343,28 +371,41 @@ uint32_t grid_for(uint32_t n_tiles, uint32_t wpb)
     return b ? b : 1u;
 }
 
+template <int W, bool FUSE>
+hipError_t launch_myers_w(const idb_pair_args &a, dim3 grid, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
+{
+    if (shm > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_pairs_myers<W, FUSE>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_pairs_myers<W, FUSE>), grid, block, shm, s, a, wpb);
+    return hipGetLastError();
+}
+
 template <bool FUSE>
 hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
 {
     if (a.n_tiles == 0) return hipSuccess;
-    const size_t per_wave = (size_t)a.K * W * 64 * sizeof(uint32_t);
-    uint32_t wpb = (uint32_t)(65536 / per_wave);
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, W);
+    const size_t limit = idb_lds_limit();
+    uint32_t wpb = (uint32_t)(std::min<size_t>(limit, 65536) / per_wave);
     if (wpb > 4) wpb = 4;
+    if (wpb == 0) wpb = per_wave <= limit ? 1u : 0u;
     if (wpb == 0) return hipErrorInvalidValue;
     const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
     const size_t shm = per_wave * wpb;
     switch (W) {
-    case 1: hipLaunchKernelGGL((k_pairs_myers<1, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 2: hipLaunchKernelGGL((k_pairs_myers<2, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 3: hipLaunchKernelGGL((k_pairs_myers<3, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 4: hipLaunchKernelGGL((k_pairs_myers<4, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 5: hipLaunchKernelGGL((k_pairs_myers<5, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 6: hipLaunchKernelGGL((k_pairs_myers<6, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 7: hipLaunchKernelGGL((k_pairs_myers<7, FUSE>), grid, block, shm, s, a, wpb); break;
-    case 8: hipLaunchKernelGGL((k_pairs_myers<8, FUSE>), grid, block, shm, s, a, wpb); break;
+    case 1: return launch_myers_w<1, FUSE>(a, grid, block, shm, wpb, s);
+    case 2: return launch_myers_w<2, FUSE>(a, grid, block, shm, wpb, s);
+    case 3: return launch_myers_w<3, FUSE>(a, grid, block, shm, wpb, s);
+    case 4: return launch_myers_w<4, FUSE>(a, grid, block, shm, wpb, s);
+    case 5: return launch_myers_w<5, FUSE>(a, grid, block, shm, wpb, s);
+    case 6: return launch_myers_w<6, FUSE>(a, grid, block, shm, wpb, s);
+    case 7: return launch_myers_w<7, FUSE>(a, grid, block, shm, wpb, s);
+    case 8: return launch_myers_w<8, FUSE>(a, grid, block, shm, wpb, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template <int BITS, bool FUSE>
@@ -390,6 +431,25 @@ hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
 }
 
 }  // namespace
+
+size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
+{
+    return ((size_t)K * W * 64 + 2 * IDB_TILE) * sizeof(uint32_t);
+}
+
+size_t idb_lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0) {
+        (void)hipGetLastError();
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+            (void)hipGetLastError();
+            v = 65536;
+        }
+    }
+    return (size_t)v;
+}
 
 hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<false>(a, W, s); }
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<true>(a, W, s); }

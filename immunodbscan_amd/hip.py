@@ -34,6 +34,11 @@ def _check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed with hipError {rc}")
 
 
+def copy_device(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int) -> None:
+    """Device-to-device copy of nbytes (runs the runtime's copy kernel)."""
+    _check(_hip.hipMemcpy(dst._p, src._p, int(nbytes), 3), "hipMemcpy D2D")
+
+
 def synchronize() -> None:
     _check(_hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
