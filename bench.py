@@ -39,6 +39,20 @@ def myers_ops_per_letter(W: int) -> int:
     return 20 * W + 4 * (W - 1)
 
 
+# band-match filter per text letter: LDS address, LDS read, AND with the band
+# mask, bit count with accumulate, mask shift-or, mask fill compare
+BAND_OPS_PER_LETTER = 6
+
+
+def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
+    W = max(1, (s["max_len"] + 31) // 32)
+    gate_ops = 2 * (vw + jw) + 2
+    ops = s["gated_pairs"] * gate_ops + s["myers_letters"] * myers_ops_per_letter(W)
+    if s["band_filter"]:
+        ops += s["gated_letters"] * BAND_OPS_PER_LETTER
+    return ops
+
+
 def cpu_baseline(rep, cl, rows: int, threads: int) -> dict:
     import numpy as np
 
@@ -124,8 +138,7 @@ def main() -> int:
     log(f"timed {args.steps} steps: {ms:.3f} ms/step; stages {s['stage_ms']}")
 
     W = max(1, (s["max_len"] + 31) // 32)
-    gate_ops = 2 * (rep.vw + rep.jw) + 2
-    ops = s["gated_pairs"] * gate_ops + s["gated_letters"] * myers_ops_per_letter(W)
+    ops = count_kernel_ops(s, rep.vw, rep.jw)
     kms = float(np.mean(cnt_ms))
     achieved = ops / (kms * 1e-3) / 1e12
     out = {
@@ -151,13 +164,16 @@ def main() -> int:
         },
         "seqs_clustered_per_sec": n / (ms * 1e-3),
         "gated_pairs_per_sec": s["gated_pairs"] / (ms * 1e-3),
+        "gated_pairs": s["gated_pairs"],
+        "myers_pairs": s["myers_pairs"],
+        "neighbour_pairs": s["neighbour_pairs"],
         "clusters": s["n_clusters"],
         "uniques": s["n_uniques"],
         "cores": s["n_cores"],
         "noise": s["n_noise"],
         "stage_ms": s["stage_ms"],
         "roofline": {
-            "kernel": f"k_pairs_myers<{W},false> (count pass)",
+            "kernel": f"k_pairs_myers<{W},false,{'true' if s['band_filter'] else 'false'}> (count pass)",
             "bound": "valu",
             "achieved": achieved,
             "peak": VALU_PEAK_TOPS,

@@ -94,13 +94,15 @@ class IdbStats(C.Structure):
         ("alphabet_size", C.c_uint32),
         ("n_buckets", C.c_uint32),
         ("fuse_from_edges", C.c_uint32),
-        ("reserved_", C.c_uint32),
+        ("band_filter", C.c_uint32),
         ("bucket_entries", C.c_uint64),
         ("tiles", C.c_uint64),
         ("gated_pairs", C.c_uint64),
         ("neighbour_pairs", C.c_uint64),
         ("fuse_pairs", C.c_uint64),
         ("gated_letters", C.c_uint64),
+        ("myers_pairs", C.c_uint64),
+        ("myers_letters", C.c_uint64),
         ("count_kernel_ms", C.c_float),
         ("fuse_kernel_ms", C.c_float),
         ("tree_depth", C.c_uint32 * 51),

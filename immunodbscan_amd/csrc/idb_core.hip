@@ -864,6 +864,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.gj = gj;
         a.gvj = gvj;
         a.single_bucket = single;
+        // The band filter pays off where random letter matches are rare
+        // (protein alphabets); with 4 letters most pairs pass it anyway.
+        a.band_filter = (!ham && W == 1 && K >= 8) ? 1u : 0u;
         a.tlut = tlut_c;
         a.tlut_f = tlut_f;
         a.cnt = cnt;
@@ -971,6 +974,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->neighbour_pairs = wk[1];
         st->fuse_pairs = wk[2];
         st->gated_letters = wk[3];
+        st->myers_pairs = wk[5];
+        st->myers_letters = wk[6];
+        st->band_filter = a.band_filter;
         st->fuse_from_edges = from_edges ? 1u : 0u;
         if (E > 0 && T > 0) {
             (void)hipEventElapsedTime(&st->count_kernel_ms, c->kev[0], c->kev[1]);

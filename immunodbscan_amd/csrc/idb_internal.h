@@ -48,20 +48,27 @@ struct idb_pair_args {
     uint32_t gj;                  // J genes per bucket row (32*jw)
     uint32_t gvj;                 // genes pairs per repertoire (32*vw*gj)
     uint32_t single_bucket;       // 1: every pair is gated explicitly
+    uint32_t band_filter;         // 1: band-match filter before Myers (single-word patterns)
     const int16_t *tlut;          // threshold by max length (count or fuse pass)
     const int16_t *tlut_f;        // fuse-pass threshold (edges recorded by the count pass)
     int32_t *cnt;                 // count pass output
     const int32_t *core;          // fuse pass input (per row)
     uint32_t *par;                // fuse pass: union-find parents
     uint32_t *bmin;               // fuse pass: border claims
-    unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] text letters
+    unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] gated letters,
+                                  // [4] edges, [5] Myers pairs, [6] Myers letters (unfiltered path)
     uint2 *edges;                 // count pass: neighbour pairs (x, y), may be null
     unsigned long long edge_cap;
     unsigned long long *n_edges;
 };
 
-// LDS bytes one wave of the Myers kernels needs, and the device's limit per block.
-size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W);
+// LDS bytes one wave of the Myers kernels needs: Peq columns, the edge
+// staging slots and the Myers queue (2 * IDB_TILE 16-bit entries).
+__host__ __device__ inline size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
+{
+    return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE) * sizeof(uint32_t);
+}
+// The device's LDS limit per block (opt-in value where the device has one).
 size_t idb_lds_limit();
 
 // Kernel launchers (defined in idb_pairs.hip).

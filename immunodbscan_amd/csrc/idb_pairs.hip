@@ -40,6 +40,13 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -150,18 +157,91 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
+// Hit bookkeeping shared by the per-query and the batched Myers paths.
+template <bool FUSE>
+__device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &est, bool hit, uint32_t x, uint32_t y,
+                                           uint32_t cx, uint32_t cy, int32_t core_x, int32_t core_y, uint32_t lane)
+{
+    if (!FUSE) {
+        if (hit) {
+            atomicAdd(a.cnt + x, (int32_t)cy);
+            atomicAdd(a.cnt + y, (int32_t)cx);
+        }
+        stage_edges(a, est, hit, x, y, lane);
+    } else if (hit) {
+        if (core_x && core_y) uf_union(a.par, x, y);
+        else if (core_x) atomicMin(a.bmin + y, x);
+        else atomicMin(a.bmin + x, y);
+    }
+}
+
+// Myers over up to 64 queued (query, target) pairs of the current tile, one
+// pair per lane.  The target's Peq column belongs to lane tl; its per-lane
+// values are fetched with cross-lane reads.
 template <int W, bool FUSE>
+__device__ __noinline__ void myers_batch(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
+                                         const uint16_t *queue, uint32_t n, uint32_t lane, uint32_t y, uint32_t m,
+                                         uint32_t cy, int32_t core_y, EdgeStage &est, unsigned long long &hits,
+                                         unsigned long long &mlet)
+{
+    const bool act = lane < n;
+    const uint32_t e = act ? queue[lane] : 0u;
+    const uint32_t qi = e >> 8, tl = e & 0xFFu;
+    const uint32_t ty = __shfl(y, tl, 64), tm = __shfl(m, tl, 64), tcy = __shfl(cy, tl, 64);
+    const int32_t tcore = __shfl(core_y, tl, 64);
+    const uint32_t q = tile.q0 + qi;
+    uint32_t x = 0, lx = 0, cx = 0;
+    int32_t core_x = 0;
+    if (act) {
+        x = a.e_row[q];
+        lx = a.e_len[q];
+        cx = a.e_cp[q];
+        if (FUSE) core_x = a.core[x];
+    }
+    const int32_t t = act ? (int32_t)a.tlut[lx > tm ? lx : tm] : -1;
+    mlet += lx;
+    idb_myers<W> st;
+    idb_myers_init<W>(st, tm);
+    const uint32_t top_w = tm ? (tm - 1) >> 5 : 0xFFFFFFFFu, top_b = tm ? (tm - 1) & 31 : 0u;
+    const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
+    for (uint32_t j = 0; j < lx; j += 4) {
+        const uint32_t word = xc[j >> 2];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (j + k < lx) {
+                const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                uint32_t eq[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + tl];
+                idb_myers_step<W>(st, eq, top_w, top_b);
+            }
+        }
+    }
+    const int32_t dist = tm ? st.score : (int32_t)lx;
+    const bool hit = act && dist <= t;
+    if (!FUSE) hits += __popcll(__ballot(hit));
+    record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane);
+}
+
+// FILTER (single-word patterns): before Myers, count the letter matches that
+// lie on the diagonal band |i - j| <= t.  Every match column of an alignment
+// with at most t edits lies on that band and there are at least
+// max(lx, ly) - t of them, so pairs with fewer band matches are not
+// neighbours.  The count costs one LDS read, an AND and a bit count per text
+// letter; surviving pairs are queued in LDS and verified by Myers 64 at a time.
+template <int W, bool FUSE, bool FILTER>
 __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t K = a.K;
-    uint32_t *peq = lds + (size_t)wib * (K * W * 64 + 2 * IDB_TILE);
+    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
     EdgeStage est{(uint2 *)(peq + K * W * 64), 0u};
+    uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
-    unsigned long long gated = 0, hits = 0, letters = 0;
+    unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
         const idb_tile tile = a.tiles[ti];
@@ -197,6 +277,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
         int32_t acc_y = 0;
+        uint32_t qlen = 0;  // queued pairs (FILTER)
 
         for (uint32_t qi = 0; qi < qn; ++qi) {
             const uint32_t q = tile.q0 + qi;
@@ -214,11 +295,50 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             const unsigned long long vm = __ballot(valid);
             if (!vm) continue;
             gated += __popcll(vm);
-            letters += (unsigned long long)__popcll(vm) * lx;
+            gletters += (unsigned long long)__popcll(vm) * lx;
+            const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
 
+            if (FILTER) {
+                // band mask for text letter j: pattern bits max(0, j-t) .. j+t
+                const int32_t need = (int32_t)L - t;
+                const int32_t tb = t < 0 ? 0 : t;
+                uint32_t band = tb >= 31 ? 0xFFFFFFFFu : ((2u << tb) - 1u);
+                uint32_t matches = 0;
+                for (uint32_t j = 0; j < lx; j += 4) {
+                    const uint32_t word = xc[j >> 2];
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) {
+                        if (j + k < lx) {
+                            const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                            matches += __builtin_popcount(peq[c * 64 + lane] & band);
+                            band = (band << 1) | ((int32_t)(j + k) < tb ? 1u : 0u);
+                        }
+                    }
+                }
+                const bool pass = valid && (int32_t)matches >= need;
+                const unsigned long long pm = __ballot(pass);
+                if (!pm) continue;
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                if (pass) queue[qlen + below] = (uint16_t)((qi << 8) | lane);
+                qlen += (uint32_t)__popcll(pm);
+                mpairs += __popcll(pm);
+                if (qlen >= IDB_TILE) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, y, m, cy, core_y, est, hits, mletters);
+                    const uint32_t rest = qlen - IDB_TILE;
+                    const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    if (lane < rest) queue[lane] = moved;
+                    qlen = rest;
+                }
+                continue;
+            }
+
+            mpairs += __popcll(vm);
+            mletters += valid ? lx : 0u;
             idb_myers<W> st;
             idb_myers_init<W>(st, m);
-            const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
             for (uint32_t j = 0; j < lx; j += 4) {
                 const uint32_t word = xc[j >> 2];
 #pragma unroll
@@ -248,14 +368,21 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 else atomicMin(a.bmin + x, y);
             }
         }
+        if (FILTER && qlen) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, y, m, cy, core_y, est, hits, mletters);
+        }
         if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
     }
     if (!FUSE) flush_edges(a, est, lane);
+    mletters = wave_sum_u64(mletters);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
             atomicAdd(a.work + 1, hits);
-            atomicAdd(a.work + 3, letters);
+            atomicAdd(a.work + 3, gletters);
+            atomicAdd(a.work + 5, mpairs);
+            atomicAdd(a.work + 6, mletters);
         }
     }
 }
@@ -371,15 +498,15 @@ uint32_t grid_for(uint32_t n_tiles, uint32_t wpb)
     return b ? b : 1u;
 }
 
-template <int W, bool FUSE>
+template <int W, bool FUSE, bool FILTER>
 hipError_t launch_myers_w(const idb_pair_args &a, dim3 grid, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
 {
     if (shm > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_pairs_myers<W, FUSE>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_pairs_myers<W, FUSE, FILTER>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_pairs_myers<W, FUSE>), grid, block, shm, s, a, wpb);
+    hipLaunchKernelGGL((k_pairs_myers<W, FUSE, FILTER>), grid, block, shm, s, a, wpb);
     return hipGetLastError();
 }
 
@@ -395,15 +522,16 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
     if (wpb == 0) return hipErrorInvalidValue;
     const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
     const size_t shm = per_wave * wpb;
+    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, true>(a, grid, block, shm, wpb, s);
     switch (W) {
-    case 1: return launch_myers_w<1, FUSE>(a, grid, block, shm, wpb, s);
-    case 2: return launch_myers_w<2, FUSE>(a, grid, block, shm, wpb, s);
-    case 3: return launch_myers_w<3, FUSE>(a, grid, block, shm, wpb, s);
-    case 4: return launch_myers_w<4, FUSE>(a, grid, block, shm, wpb, s);
-    case 5: return launch_myers_w<5, FUSE>(a, grid, block, shm, wpb, s);
-    case 6: return launch_myers_w<6, FUSE>(a, grid, block, shm, wpb, s);
-    case 7: return launch_myers_w<7, FUSE>(a, grid, block, shm, wpb, s);
-    case 8: return launch_myers_w<8, FUSE>(a, grid, block, shm, wpb, s);
+    case 1: return launch_myers_w<1, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 2: return launch_myers_w<2, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 3: return launch_myers_w<3, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 4: return launch_myers_w<4, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 5: return launch_myers_w<5, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 6: return launch_myers_w<6, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 7: return launch_myers_w<7, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 8: return launch_myers_w<8, FUSE, false>(a, grid, block, shm, wpb, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -431,11 +559,6 @@ hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
 }
 
 }  // namespace
-
-size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
-{
-    return ((size_t)K * W * 64 + 2 * IDB_TILE) * sizeof(uint32_t);
-}
 
 size_t idb_lds_limit()
 {

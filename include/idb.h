@@ -89,12 +89,14 @@ typedef struct {
     uint32_t n_rows, n_uniques, n_cores, n_clusters, n_noise;
     uint32_t max_len, alphabet_size, n_buckets;
     uint32_t fuse_from_edges;     /* 1: clusters grown from the count pass's pair list       */
-    uint32_t reserved_;
+    uint32_t band_filter;         /* 1: band-match filter ran before Myers                   */
     uint64_t bucket_entries, tiles;
     uint64_t gated_pairs;         /* pairs reaching a distance kernel (count pass)   */
     uint64_t neighbour_pairs;     /* unordered unique pairs with N(x,y)              */
     uint64_t fuse_pairs;          /* pairs evaluated by the fuse pass                */
-    uint64_t gated_letters;       /* text letters consumed by the count pass's gated pairs */
+    uint64_t gated_letters;       /* text letters of the count pass's gated pairs    */
+    uint64_t myers_pairs;         /* gated pairs that reached Myers (after the band filter) */
+    uint64_t myers_letters;       /* text letters those Myers runs consumed          */
     float count_kernel_ms;        /* count-pass kernel, HIP events on the library stream */
     float fuse_kernel_ms;         /* fuse-pass kernel                                */
     uint32_t tree_depth[51];      /* union-find depth histogram when tree_stats      */
