@@ -39,9 +39,9 @@ def myers_ops_per_letter(W: int) -> int:
     return 20 * W + 4 * (W - 1)
 
 
-# band-match filter per text letter: LDS address, LDS read, AND with the band
-# mask, bit count with accumulate, mask shift-or, mask fill compare
-BAND_OPS_PER_LETTER = 6
+# absent-letter filter per pair: per direction two AND-NOTs of the presence
+# planes and two bit counts, then a maximum and the threshold compare
+FILTER_OPS_PER_PAIR = 10
 
 
 def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
@@ -49,7 +49,7 @@ def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
     gate_ops = 2 * (vw + jw) + 2
     ops = s["gated_pairs"] * gate_ops + s["myers_letters"] * myers_ops_per_letter(W)
     if s["band_filter"]:
-        ops += s["gated_letters"] * BAND_OPS_PER_LETTER
+        ops += s["gated_pairs"] * FILTER_OPS_PER_PAIR
     return ops
 
 

@@ -39,7 +39,7 @@ enum Slot {
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
-    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES,
+    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT,
     S_COUNT
 };
 
@@ -306,22 +306,27 @@ __global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const 
     }
 }
 
-__device__ __forceinline__ bool has_bit_below(const int32_t *w, uint32_t nw, uint32_t g)
+// Residues mod 16 of the set bits of `w` below gene `g` (all bits when g is
+// ~0u), folded into a 16-bit mask.
+__device__ __forceinline__ uint32_t gene_residues(const int32_t *w, uint32_t nw, uint32_t g)
 {
+    uint32_t r = 0;
     for (uint32_t k = 0; k < nw && k * 32 < g; ++k) {
         uint32_t m = (uint32_t)w[k];
         if (k * 32 + 32 > g) m &= (1u << (g - k * 32)) - 1u;
-        if (m) return true;
+        r |= (m | (m >> 16)) & 0xFFFFu;
     }
-    return false;
+    return r;
 }
 
-// Materialise entries in bucket order: row, length, copies, check flag, codes.
+// Materialise entries in bucket order: the 32-byte record, the length (for
+// the tile builder) and the letters as codes or packed words.
 __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint32_t E, const uint8_t *seq,
                                const uint64_t *off, const uint32_t *copies, const int32_t *vb, uint32_t vw,
                                const int32_t *jb, uint32_t jw, uint32_t single, uint32_t gj, uint32_t gvj,
-                               const uint8_t *cmap, uint32_t S, uint32_t HW, uint32_t bits, uint16_t *elen,
-                               uint32_t *ecp, uint8_t *echk, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
+                               const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
+                               const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint16_t *elen,
+                               idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
 {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const uint64_t k = ekey[e];
@@ -329,23 +334,43 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         const uint32_t L = (uint32_t)(k & 0xFFFFu);
         const uint64_t bucket = k >> 16;
         elen[e] = (uint16_t)L;
-        ecp[e] = copies[u];
         bflag[e] = (e == 0) || ((ekey[e - 1] >> 16) != bucket);
+        idb_entry r;
+        r.row = u;
+        r.cp = copies[u];
+        uint32_t sv, sj;
         if (single) {
-            echk[e] = 1;
+            sv = gene_residues(vb + (size_t)u * vw, vw, 0xFFFFFFFFu);
+            sj = gene_residues(jb + (size_t)u * jw, jw, 0xFFFFFFFFu);
         } else {
             const uint32_t bl = (uint32_t)(bucket % gvj);
-            const uint32_t v = bl / gj, j = bl % gj;
-            echk[e] = has_bit_below(vb + (size_t)u * vw, vw, v) || has_bit_below(jb + (size_t)u * jw, jw, j);
+            sv = gene_residues(vb + (size_t)u * vw, vw, bl / gj);
+            sj = gene_residues(jb + (size_t)u * jw, jw, bl % gj);
         }
+        const uint32_t chk = single ? 1u : ((sv | sj) != 0u);
+        r.len_chk = L | (chk << 16);
+        r.t2 = (uint32_t)(uint16_t)tlut_c[L] | ((uint32_t)(uint16_t)tlut_f[L] << 16);
+        r.sig = sv | (sj << 16);
         const uint8_t *s = seq + off[u];
+        uint32_t p1 = 0, p2 = 0;
+        if (pres)
+            for (uint32_t pos = 0; pos < L; ++pos) {
+                const uint32_t bit = 1u << cmap[s[pos]];
+                p2 |= p1 & bit;
+                p1 |= bit;
+            }
+        r.pres1 = p1;
+        r.pres2 = p2;
+        r.pad = 0;
+        ent[e] = r;
         if (ecodes) {
+            // letters past the row's end carry the sentinel code `pad`
             uint32_t *dst = ecodes + (size_t)e * (S / 4);
             for (uint32_t w = 0; w < S / 4; ++w) {
                 uint32_t word = 0;
                 for (uint32_t b = 0; b < 4; ++b) {
                     const uint32_t pos = w * 4 + b;
-                    if (pos < L) word |= (uint32_t)cmap[s[pos]] << (8 * b);
+                    word |= (pos < L ? (uint32_t)cmap[s[pos]] : pad) << (8 * b);
                 }
                 dst[w] = word;
             }
@@ -718,10 +743,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     } else {
         W = std::max(1u, (maxlen + 31) / 32);
         if (W > IDB_MAX_W) fail(c, IDB_ERR_UNSUPPORTED, "Levenshtein rows longer than 256 letters are not supported");
-        if (idb_myers_lds_per_wave(std::max(K, 1u), W) > idb_lds_limit())
+        if (idb_myers_lds_per_wave(std::max(std::min(K + 1, 256u), 1u), W) > idb_lds_limit())
             fail(c, IDB_ERR_UNSUPPORTED, "alphabet of " + std::to_string(K) + " letters with rows of " +
                                              std::to_string(maxlen) + " letters exceeds the LDS budget");
-        S = std::max(4u, (maxlen + 3) / 4 * 4);
+        S = std::max(16u, (maxlen + 15) / 16 * 16);
     }
     sort_pairs_u64(c, key, key2, idx, idx2, N, 64);
     uint32_t *head0 = buf<uint32_t>(c, S_HEAD, N);
@@ -819,13 +844,20 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
         sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, 16 + bits_for(max_bucket));
         uint16_t *elen = buf<uint16_t>(c, S_ELEN, E);
-        uint32_t *ecp = buf<uint32_t>(c, S_ECP, E);
-        uint8_t *echk = buf<uint8_t>(c, S_ECHK, E);
+        idb_entry *ent = buf<idb_entry>(c, S_ENT, E);
         uint8_t *bflag = buf<uint8_t>(c, S_BFLAG, E);
         uint32_t *ecodes = ham ? nullptr : buf<uint32_t>(c, S_ECODES, (size_t)E * (S / 4));
         uint32_t *epk = ham ? buf<uint32_t>(c, S_EPK, (size_t)E * HW) : nullptr;
+        // Levenshtein codes are padded with a sentinel letter K whose Peq row
+        // stays zero (with all 256 byte values present code 0 is used; no
+        // kernel reads letters past a row's end)
+        const uint32_t pad = (!ham && K < 256) ? K : 0u;
+        // The absent-letter filter pays off where random letter matches are
+        // rare (protein alphabets); with 4 letters nearly every pair passes.
+        const bool filt = !ham && W == 1 && K >= 8 && K <= 31;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
-                           jb, jw, single, gj, gvj, cmap, S, HW, bits, elen, ecp, echk, ecodes, epk, bflag);
+                           jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u, elen, ent,
+                           ecodes, epk, bflag);
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
         uint32_t *qbcnt = buf<uint32_t>(c, S_QBCNT, (size_t)E + 1);
@@ -847,28 +879,20 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
                            banded ? 1u : 0u, ekey2, gvj, toff, tiles);
         a.tiles = tiles;
         a.n_tiles = T;
-        a.e_row = erow2;
-        a.e_len = elen;
-        a.e_cp = ecp;
-        a.e_chk = echk;
+        a.ent = ent;
         a.e_codes = (const uint8_t *)ecodes;
         a.e_pk = epk;
         a.S = S;
         a.HW = HW;
-        a.K = std::max(K, 1u);
+        a.K = ham ? std::max(K, 1u) : std::max(std::min(K + 1, 256u), 1u);
         a.bits = bits;
         a.vbits = vb;
         a.jbits = jb;
         a.vw = vw;
         a.jw = jw;
         a.gj = gj;
-        a.gvj = gvj;
         a.single_bucket = single;
-        // The band filter pays off where random letter matches are rare
-        // (protein alphabets); with 4 letters most pairs pass it anyway.
-        a.band_filter = (!ham && W == 1 && K >= 8) ? 1u : 0u;
-        a.tlut = tlut_c;
-        a.tlut_f = tlut_f;
+        a.band_filter = filt ? 1u : 0u;
         a.cnt = cnt;
         a.work = work;
         // Neighbour pairs found by the count pass are kept for the fuse step;
@@ -897,7 +921,6 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         if (from_edges) {
             check(c, idb_launch_union_edges(a.edges, n_edges, core, par, bmin, s), "union kernel launch");
         } else {
-            a.tlut = tlut_f;
             a.core = core;
             a.par = par;
             a.bmin = bmin;

@@ -9,9 +9,10 @@
 //             par[u32]  union-find parent over rows, bmin[u32] border claim
 //   entries:  one per (unique row, V gene, J gene) bucket membership, sorted by
 //             (repertoire, V gene, J gene, length, row); a bucket is a
-//             contiguous run.  Per entry: row, length, copies, check flag and
-//             the row's letters as dense 8-bit codes (stride S bytes) or, for
-//             Hamming mode, packed words (stride HW words).
+//             contiguous run.  Per entry a 32-byte record (idb_entry, loaded
+//             in one round trip) and the row's letters as dense 8-bit codes
+//             (stride S bytes, S a multiple of 16) or, for Hamming mode,
+//             packed words (stride HW words).
 //   tiles:    (query block, target block) pairs of <= 64 entries each inside a
 //             bucket whose length band can hold a neighbour pair.
 #pragma once
@@ -27,18 +28,29 @@
 struct idb_tile {
     uint32_t q0;                  // first query entry
     uint32_t t0;                  // first target entry
-    uint32_t bucket;              // bucket id (gene pair) for the canonical check
+    uint32_t bucket;              // bucket id (gene pair) within its repertoire
     uint32_t shape;               // qn | tn << 8 | diag << 16
+};
+
+// Everything the pair kernels need about one entry, in one 32-byte record.
+struct __align__(16) idb_entry {
+    uint32_t row;                 // input row (a unique representative)
+    uint32_t cp;                  // copies of the row
+    uint32_t len_chk;             // length | chk << 16 (chk: the row may share a lower gene,
+                                  // or every row in single-bucket mode)
+    uint32_t t2;                  // threshold at this length: count pass (int16, low half),
+                                  // fuse pass (int16, high half)
+    uint32_t sig;                 // residues mod 16 of the genes the canonical test looks at:
+                                  // V in the low half, J in the high half
+    uint32_t pres1, pres2;        // letters present once / twice (absent-letter filter)
+    uint32_t pad;
 };
 
 // Uniform parameters of a pair-kernel launch.
 struct idb_pair_args {
     const idb_tile *tiles;
     uint32_t n_tiles;
-    const uint32_t *e_row;
-    const uint16_t *e_len;
-    const uint32_t *e_cp;         // copies of the entry's row
-    const uint8_t *e_chk;         // 1 when the row may share a lower gene
+    const idb_entry *ent;
     const uint8_t *e_codes;       // Myers: stride S bytes of codes
     const uint32_t *e_pk;         // Hamming: stride HW packed words
     uint32_t S, HW, K;
@@ -46,17 +58,14 @@ struct idb_pair_args {
     const int32_t *vbits, *jbits;
     uint32_t vw, jw;
     uint32_t gj;                  // J genes per bucket row (32*jw)
-    uint32_t gvj;                 // genes pairs per repertoire (32*vw*gj)
     uint32_t single_bucket;       // 1: every pair is gated explicitly
-    uint32_t band_filter;         // 1: band-match filter before Myers (single-word patterns)
-    const int16_t *tlut;          // threshold by max length (count or fuse pass)
-    const int16_t *tlut_f;        // fuse-pass threshold (edges recorded by the count pass)
+    uint32_t band_filter;         // 1: absent-letter filter before Myers (alphabets <= 31 letters)
     int32_t *cnt;                 // count pass output
     const int32_t *core;          // fuse pass input (per row)
     uint32_t *par;                // fuse pass: union-find parents
     uint32_t *bmin;               // fuse pass: border claims
     unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] gated letters,
-                                  // [4] edges, [5] Myers pairs, [6] Myers letters (unfiltered path)
+                                  // [4] edges, [5] Myers pairs, [6] Myers letters
     uint2 *edges;                 // count pass: neighbour pairs (x, y), may be null
     unsigned long long edge_cap;
     unsigned long long *n_edges;

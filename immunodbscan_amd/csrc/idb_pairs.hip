@@ -9,21 +9,24 @@
 // query entries of one V/J bucket:
 //   * each lane holds one target (the Myers pattern) and builds its match
 //     masks Peq[letter][word] in its own LDS column ([letter][word][lane]
-//     layout, so a wave's 64 lanes hit 64 consecutive dwords: no bank
+//     layout: a wave's 64 lanes touch 64 consecutive dwords, no bank
 //     conflicts);
-//   * the queries are the Myers text and are wave-uniform: their letters come
-//     through scalar loads, and the wave walks them one after another;
+//   * lane l also holds query l's 32-byte entry record for the whole tile; the
+//     query loop broadcasts it with readlane, so the loop itself issues no
+//     memory loads in the common case (one round trip per tile fetches both
+//     records; code words come in 16-byte loads);
 //   * gate work per pair is the length band, the triangle order inside the
-//     diagonal tile and, only for rows with ambiguous calls, the canonical
-//     bucket test (lowest common V gene and J gene equal the bucket's), so
-//     each unordered pair is evaluated in exactly one bucket.
-// Count pass: copies-weighted counts, accumulated in registers and flushed
-// with one atomic per target per tile and one per query hit.  Fuse pass: pairs
-// with at least one core; core-core pairs are joined in a lock-free
-// union-find that links the larger root under the smaller (roots are
-// component minima), and a non-core row records its minimum-index core
-// neighbour with atomicMin (the canonical form of the reference's first-come
-// atomicCAS at :113-122).
+//     diagonal tile and, only when both rows carry a gene below the bucket's,
+//     the canonical bucket test (lowest common V gene and J gene equal the
+//     bucket's), so each unordered pair is evaluated in exactly one bucket.
+//     Gene residues mod 16 decide most of those tests without loading the
+//     bitsets.
+// Count pass: copies-weighted counts and the neighbour pairs (kept for the
+// fuse step).  Fuse pass (fallback when the pair list overflows): pairs with at
+// least one core; core-core pairs are joined in a lock-free union-find that
+// links the larger root under the smaller (roots are component minima), and a
+// non-core row records its minimum-index core neighbour with atomicMin (the
+// canonical form of the reference's first-come atomicCAS at :113-122).
 #include <algorithm>
 
 #include "idb_internal.h"
@@ -47,6 +50,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane) { return __builtin_amdgcn_readlane(v, lane); }
+
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -67,8 +72,8 @@ __device__ uint32_t uf_find(uint32_t *par, uint32_t x)
     return x;
 }
 
-// Link the larger root under the smaller one.  Every retry continues from a
-// strictly smaller index, so the loop ends.
+// Link the larger root under the smaller one.  After a failed exchange the
+// larger of the two roots strictly decreases, so the loop ends.
 __device__ void uf_union(uint32_t *par, uint32_t a, uint32_t b)
 {
     a = uf_find(par, a);
@@ -97,8 +102,7 @@ __device__ __forceinline__ int lowest_common(const int32_t *a, const int32_t *b,
     return -1;
 }
 
-// Canonical-bucket test for a pair of rows that both carry more than the
-// bucket's genes (or every pair, in single-bucket mode, where it is the V/J
+// Exact canonical-bucket test from the bitsets (single-bucket mode: the V/J
 // gate of cuda/BitsetVector.inl:214-237).
 __device__ bool pair_in_bucket(const idb_pair_args &a, uint32_t x, uint32_t y, uint32_t bucket)
 {
@@ -106,8 +110,25 @@ __device__ bool pair_in_bucket(const idb_pair_args &a, uint32_t x, uint32_t y, u
     const int32_t *jx = a.jbits + (size_t)x * a.jw, *jy = a.jbits + (size_t)y * a.jw;
     const int lv = lowest_common(vx, vy, a.vw), lj = lowest_common(jx, jy, a.jw);
     if (a.single_bucket) return lv >= 0 && lj >= 0;
-    const uint32_t bl = bucket % a.gvj;
-    return lv == (int)(bl / a.gj) && lj == (int)(bl % a.gj);
+    return lv == (int)(bucket / a.gj) && lj == (int)(bucket % a.gj);
+}
+
+// The signature holds the residues mod 16 of the genes in question (canonical
+// mode: genes below the bucket's; single-bucket mode: all genes).  Canonical
+// mode: no common V residue and no common J residue -> no common lower gene ->
+// canonical.  Single-bucket mode: no common V (or J) residue -> no common gene
+// -> the gate fails.  Otherwise the bitsets decide.
+__device__ __forceinline__ bool gate_pair(const idb_pair_args &a, bool chk_x, bool chk_y, uint32_t sx, uint32_t sy,
+                                          uint32_t x, uint32_t y, uint32_t bucket)
+{
+    if (!(a.single_bucket || (chk_x && chk_y))) return true;
+    const uint32_t c = sx & sy;
+    if (a.single_bucket) {
+        if ((c & 0xFFFFu) == 0u || (c >> 16) == 0u) return false;
+    } else if (c == 0u) {
+        return true;
+    }
+    return pair_in_bucket(a, x, y, bucket);
 }
 
 // Neighbour pairs are staged per wave in LDS (IDB_TILE entries) and appended
@@ -148,16 +169,43 @@ __device__ __forceinline__ void stage_edges(const idb_pair_args &a, EdgeStage &s
     st.n += cnt;
 }
 
+// The grid holds only as many waves as can be resident at once (no partly
+// empty second round of waves trails the launch); wave w takes tiles w,
+// w + waves, w + 2 waves, ...  A shared tile counter balances better in
+// principle but costs a device-scope atomic round trip per tile, more than
+// the imbalance it removes.
+__device__ __forceinline__ uint32_t next_tile(uint32_t ti, uint32_t total_waves) { return ti + total_waves; }
+
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
-// walks a contiguous run of tiles (neighbouring tiles share query blocks,
-// which then stay in that XCD's L2).
+// starts on a contiguous run of tiles (neighbouring tiles share query blocks).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
 {
     const uint32_t per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
     return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
-// Hit bookkeeping shared by the per-query and the batched Myers paths.
+__device__ __forceinline__ int32_t t_of(uint32_t t2, bool fuse)
+{
+    return fuse ? (int32_t)(int16_t)(t2 >> 16) : (int32_t)(int16_t)(t2 & 0xFFFFu);
+}
+
+__device__ __forceinline__ idb_entry load_entry(const idb_entry *e)
+{
+    const uint4 *p = (const uint4 *)e;
+    const uint4 a = p[0], b = p[1];
+    idb_entry r;
+    r.row = a.x;
+    r.cp = a.y;
+    r.len_chk = a.z;
+    r.t2 = a.w;
+    r.sig = b.x;
+    r.pres1 = b.y;
+    r.pres2 = b.z;
+    r.pad = 0;
+    return r;
+}
+
+// Hit bookkeeping of the batched Myers path (one pair per lane).
 template <bool FUSE>
 __device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &est, bool hit, uint32_t x, uint32_t y,
                                            uint32_t cx, uint32_t cy, int32_t core_x, int32_t core_y, uint32_t lane)
@@ -176,34 +224,36 @@ __device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &es
 }
 
 // Myers over up to 64 queued (query, target) pairs of the current tile, one
-// pair per lane.  The target's Peq column belongs to lane tl; its per-lane
-// values are fetched with cross-lane reads.
+// pair per lane.  The target's Peq column belongs to lane tl; query and
+// target fields come from the lanes that hold them.
 template <int W, bool FUSE>
-__device__ __noinline__ void myers_batch(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
-                                         const uint16_t *queue, uint32_t n, uint32_t lane, uint32_t y, uint32_t m,
-                                         uint32_t cy, int32_t core_y, EdgeStage &est, unsigned long long &hits,
-                                         unsigned long long &mlet)
+__device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
+                                            const uint16_t *queue, uint32_t n, uint32_t lane, const idb_entry &qe,
+                                            const idb_entry &te, int32_t core_y, int32_t q_core, EdgeStage &est,
+                                            unsigned long long &hits, unsigned long long &mlet)
 {
+    // Every cross-lane read runs with the whole wave active: a cross-lane
+    // read from a lane that is switched off returns 0.
     const bool act = lane < n;
-    const uint32_t e = act ? queue[lane] : 0u;
-    const uint32_t qi = e >> 8, tl = e & 0xFFu;
-    const uint32_t ty = __shfl(y, tl, 64), tm = __shfl(m, tl, 64), tcy = __shfl(cy, tl, 64);
+    const uint32_t e = queue[lane < n ? lane : 0u];
+    const int qi = (int)(e >> 8), tl = (int)(e & 0xFFu);
+    const uint32_t ty = (uint32_t)__shfl((int)te.row, tl, 64);
+    const uint32_t tm = (uint32_t)__shfl((int)te.len_chk, tl, 64) & 0xFFFFu;
+    const uint32_t tcy = (uint32_t)__shfl((int)te.cp, tl, 64);
+    const uint32_t tt2 = (uint32_t)__shfl((int)te.t2, tl, 64);
     const int32_t tcore = __shfl(core_y, tl, 64);
-    const uint32_t q = tile.q0 + qi;
-    uint32_t x = 0, lx = 0, cx = 0;
-    int32_t core_x = 0;
-    if (act) {
-        x = a.e_row[q];
-        lx = a.e_len[q];
-        cx = a.e_cp[q];
-        if (FUSE) core_x = a.core[x];
-    }
-    const int32_t t = act ? (int32_t)a.tlut[lx > tm ? lx : tm] : -1;
+    const uint32_t x = (uint32_t)__shfl((int)qe.row, qi, 64);
+    const uint32_t lxq = (uint32_t)__shfl((int)qe.len_chk, qi, 64) & 0xFFFFu;
+    const uint32_t cx = (uint32_t)__shfl((int)qe.cp, qi, 64);
+    const uint32_t qt2 = (uint32_t)__shfl((int)qe.t2, qi, 64);
+    const int32_t core_x = __shfl(q_core, qi, 64);
+    const uint32_t lx = act ? lxq : 0u;
+    const int32_t t = !act ? -1 : (lx >= tm ? t_of(qt2, FUSE) : t_of(tt2, FUSE));
     mlet += lx;
     idb_myers<W> st;
     idb_myers_init<W>(st, tm);
     const uint32_t top_w = tm ? (tm - 1) >> 5 : 0xFFFFFFFFu, top_b = tm ? (tm - 1) & 31 : 0u;
-    const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
+    const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
     for (uint32_t j = 0; j < lx; j += 4) {
         const uint32_t word = xc[j >> 2];
 #pragma unroll
@@ -223,12 +273,9 @@ __device__ __noinline__ void myers_batch(const idb_pair_args &a, const idb_tile 
     record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane);
 }
 
-// FILTER (single-word patterns): before Myers, count the letter matches that
-// lie on the diagonal band |i - j| <= t.  Every match column of an alignment
-// with at most t edits lies on that band and there are at least
-// max(lx, ly) - t of them, so pairs with fewer band matches are not
-// neighbours.  The count costs one LDS read, an AND and a bit count per text
-// letter; surviving pairs are queued in LDS and verified by Myers 64 at a time.
+// FILTER (alphabets of at most 31 letters): before Myers, a pair must pass the
+// absent-letter bound, a few bit operations per pair; surviving pairs are
+// queued in LDS and verified by Myers 64 at a time.
 template <int W, bool FUSE, bool FILTER>
 __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
 {
@@ -243,36 +290,42 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
 
-    for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
         const idb_tile tile = a.tiles[ti];
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
         const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
-        uint32_t y = 0, m = 0, cy = 0;
-        bool chk_y = false;
-        int32_t core_y = 0;
-        if (tvalid) {
-            y = a.e_row[p];
-            m = a.e_len[p];
-            cy = a.e_cp[p];
-            chk_y = a.e_chk[p] != 0;
-            if (FUSE) core_y = a.core[y];
+        // both records of this lane in one round trip
+        idb_entry te{}, qe{};
+        if (tvalid) te = load_entry(a.ent + p);
+        if (lane < qn) qe = load_entry(a.ent + tile.q0 + lane);
+        const uint32_t y = te.row, m = te.len_chk & 0xFFFFu, cy = te.cp;
+        const bool chk_y = (te.len_chk >> 16) & 1u;
+        const int32_t t_m = tvalid ? t_of(te.t2, FUSE) : -1;
+        int32_t core_y = 0, q_core = 0;
+        if (FUSE) {
+            if (tvalid) core_y = a.core[y];
+            if (lane < qn) q_core = a.core[qe.row];
         }
-        // Peq for this lane's pattern, in its own LDS column.
+        // Peq for this lane's pattern, in its own LDS column: code words come
+        // 16 bytes at a time (S is a multiple of 16)
         for (uint32_t c = 0; c < K; ++c)
 #pragma unroll
             for (int w = 0; w < W; ++w) peq[(c * W + w) * 64 + lane] = 0u;
         if (tvalid) {
-            const uint32_t *codes = (const uint32_t *)(a.e_codes + (size_t)p * a.S);
-            for (uint32_t i = 0; i < m; i += 4) {
-                const uint32_t word = codes[i >> 2];
+            const uint4 *codes = (const uint4 *)(a.e_codes + (size_t)p * a.S);
+            for (uint32_t i0 = 0; i0 < m; i0 += 16) {
+                const uint4 v = codes[i0 >> 4];
+                const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (uint32_t k = 0; k < 4; ++k)
-                    if (i + k < m) {
-                        const uint32_t c = (word >> (8 * k)) & 0xFFu;
-                        peq[(c * W + ((i + k) >> 5)) * 64 + lane] |= 1u << ((i + k) & 31);
+                for (uint32_t k = 0; k < 16; ++k) {
+                    const uint32_t i = i0 + k;
+                    if (i < m) {
+                        const uint32_t c = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+                        peq[(c * W + (i >> 5)) * 64 + lane] |= 1u << (i & 31);
                     }
+                }
             }
         }
         const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
@@ -280,42 +333,34 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         uint32_t qlen = 0;  // queued pairs (FILTER)
 
         for (uint32_t qi = 0; qi < qn; ++qi) {
-            const uint32_t q = tile.q0 + qi;
-            const uint32_t x = a.e_row[q];
-            const uint32_t lx = a.e_len[q];
-            const bool chk_x = a.e_chk[q] != 0;
-            int32_t core_x = 0;
-            if (FUSE) core_x = a.core[x];
-            const uint32_t L = lx > m ? lx : m;
-            const int32_t t = tvalid ? (int32_t)a.tlut[L] : -1;
+            const uint32_t x = rl(qe.row, qi);
+            const uint32_t lc = rl(qe.len_chk, qi);
+            const uint32_t lx = lc & 0xFFFFu;
+            const bool chk_x = (lc >> 16) & 1u;
+            const int32_t t = !tvalid ? -1 : (lx >= m ? t_of(rl(qe.t2, qi), FUSE) : t_m);
             const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
             bool valid = tvalid && (!diag || lane > qi) && dl <= t;
-            if (FUSE) valid = valid && (core_x | core_y);
-            if (valid && (a.single_bucket || (chk_x && chk_y))) valid = pair_in_bucket(a, x, y, tile.bucket);
+            int32_t core_x = 0;
+            if (FUSE) {
+                core_x = (int32_t)rl((uint32_t)q_core, qi);
+                valid = valid && (core_x | core_y);
+            }
+            if (valid) valid = gate_pair(a, chk_x, chk_y, rl(qe.sig, qi), te.sig, x, y, tile.bucket);
             const unsigned long long vm = __ballot(valid);
             if (!vm) continue;
             gated += __popcll(vm);
             gletters += (unsigned long long)__popcll(vm) * lx;
-            const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)q * a.S);
 
             if (FILTER) {
-                // band mask for text letter j: pattern bits max(0, j-t) .. j+t
-                const int32_t need = (int32_t)L - t;
-                const int32_t tb = t < 0 ? 0 : t;
-                uint32_t band = tb >= 31 ? 0xFFFFFFFFu : ((2u << tb) - 1u);
-                uint32_t matches = 0;
-                for (uint32_t j = 0; j < lx; j += 4) {
-                    const uint32_t word = xc[j >> 2];
-#pragma unroll
-                    for (uint32_t k = 0; k < 4; ++k) {
-                        if (j + k < lx) {
-                            const uint32_t c = (word >> (8 * k)) & 0xFFu;
-                            matches += __builtin_popcount(peq[c * 64 + lane] & band);
-                            band = (band << 1) | ((int32_t)(j + k) < tb ? 1u : 0u);
-                        }
-                    }
-                }
-                const bool pass = valid && (int32_t)matches >= need;
+                // Absent-letter bound: a letter of x whose symbol never occurs
+                // in y must be substituted or deleted, one edit per position,
+                // so lev(x, y) >= #{i : x_i not in y} (and symmetrically).
+                // The planes hold the letters seen at least once / twice.
+                const uint32_t p1x = rl(qe.pres1, qi), p2x = rl(qe.pres2, qi);
+                const uint32_t lbx = __builtin_popcount(p1x & ~te.pres1) + __builtin_popcount(p2x & ~te.pres1);
+                const uint32_t lby = __builtin_popcount(te.pres1 & ~p1x) + __builtin_popcount(te.pres2 & ~p1x);
+                const uint32_t lb = lbx > lby ? lbx : lby;
+                const bool pass = valid && (int32_t)lb <= t;
                 const unsigned long long pm = __ballot(pass);
                 if (!pm) continue;
                 const uint32_t below =
@@ -325,7 +370,8 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 mpairs += __popcll(pm);
                 if (qlen >= IDB_TILE) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, y, m, cy, core_y, est, hits, mletters);
+                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, hits,
+                                         mletters);
                     const uint32_t rest = qlen - IDB_TILE;
                     const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -339,6 +385,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             mletters += valid ? lx : 0u;
             idb_myers<W> st;
             idb_myers_init<W>(st, m);
+            const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
             for (uint32_t j = 0; j < lx; j += 4) {
                 const uint32_t word = xc[j >> 2];
 #pragma unroll
@@ -358,7 +405,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (!hm) continue;
             if (!FUSE) {
                 hits += __popcll(hm);
-                if (hit) acc_y += (int32_t)a.e_cp[q];
+                if (hit) acc_y += (int32_t)rl(qe.cp, qi);
                 const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
                 if (lane == 0) atomicAdd(a.cnt + x, s);
                 stage_edges(a, est, hit, x, y, lane);
@@ -370,7 +417,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         if (FILTER && qlen) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, y, m, cy, core_y, est, hits, mletters);
+            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, hits, mletters);
         }
         if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
     }
@@ -387,6 +434,9 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     }
 }
 
+// Hamming mode: equal lengths only; packed words (2-bit codes for <= 4
+// letters, else 8-bit) compared by XOR, one bit per mismatching field and a
+// bit count.  Query words are preloaded per lane when they fit registers.
 template <int HWMAX, int BITS, bool FUSE>
 __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t wpb)
 {
@@ -397,68 +447,84 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     const uint32_t HW = a.HW;
     __shared__ uint2 stage[4][IDB_TILE];
     EdgeStage est{stage[wib], 0u};
-    unsigned long long gated = 0, hits = 0;
+    constexpr bool QREG = HWMAX <= 8;
+    unsigned long long gated = 0, hits = 0, gletters = 0;
 
-    for (uint32_t ti = wave0; ti < a.n_tiles; ti += total_waves) {
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
         const idb_tile tile = a.tiles[ti];
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
         const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
-        uint32_t y = 0, m = 0, cy = 0;
-        bool chk_y = false;
-        int32_t core_y = 0;
-        uint32_t tw[HWMAX];
+        idb_entry te{}, qe{};
+        if (tvalid) te = load_entry(a.ent + p);
+        if (lane < qn) qe = load_entry(a.ent + tile.q0 + lane);
+        uint32_t tw[HWMAX], qw[QREG ? HWMAX : 1];
 #pragma unroll
         for (int w = 0; w < HWMAX; ++w) tw[w] = 0u;
+#pragma unroll
+        for (int w = 0; w < (QREG ? HWMAX : 1); ++w) qw[w] = 0u;
         if (tvalid) {
-            y = a.e_row[p];
-            m = a.e_len[p];
-            cy = a.e_cp[p];
-            chk_y = a.e_chk[p] != 0;
-            if (FUSE) core_y = a.core[y];
             const uint32_t *pk = a.e_pk + (size_t)p * HW;
 #pragma unroll
             for (int w = 0; w < HWMAX; ++w)
                 if ((uint32_t)w < HW) tw[w] = pk[w];
         }
+        if (QREG && lane < qn) {
+            const uint32_t *pk = a.e_pk + (size_t)(tile.q0 + lane) * HW;
+#pragma unroll
+            for (int w = 0; w < (QREG ? HWMAX : 1); ++w)
+                if ((uint32_t)w < HW) qw[w] = pk[w];
+        }
+        const uint32_t y = te.row, m = te.len_chk & 0xFFFFu, cy = te.cp;
+        const bool chk_y = (te.len_chk >> 16) & 1u;
+        int32_t core_y = 0, q_core = 0;
+        if (FUSE) {
+            if (tvalid) core_y = a.core[y];
+            if (lane < qn) q_core = a.core[qe.row];
+        }
         int32_t acc_y = 0;
         for (uint32_t qi = 0; qi < qn; ++qi) {
-            const uint32_t q = tile.q0 + qi;
-            const uint32_t x = a.e_row[q];
-            const uint32_t lx = a.e_len[q];
-            const bool chk_x = a.e_chk[q] != 0;
-            int32_t core_x = 0;
-            if (FUSE) core_x = a.core[x];
-            const int32_t t = (int32_t)a.tlut[lx];
+            const uint32_t x = rl(qe.row, qi);
+            const uint32_t lc = rl(qe.len_chk, qi);
+            const uint32_t lx = lc & 0xFFFFu;
+            const bool chk_x = (lc >> 16) & 1u;
+            const uint32_t qt2 = rl(qe.t2, qi);
             bool valid = tvalid && (!diag || lane > qi) && lx == m;
-            if (FUSE) valid = valid && (core_x | core_y);
-            if (valid && (a.single_bucket || (chk_x && chk_y))) valid = pair_in_bucket(a, x, y, tile.bucket);
+            int32_t core_x = 0;
+            if (FUSE) {
+                core_x = (int32_t)rl((uint32_t)q_core, qi);
+                valid = valid && (core_x | core_y);
+            }
+            if (valid) valid = gate_pair(a, chk_x, chk_y, rl(qe.sig, qi), te.sig, x, y, tile.bucket);
             const unsigned long long vm = __ballot(valid);
             if (!vm) continue;
             gated += __popcll(vm);
+            gletters += (unsigned long long)__popcll(vm) * lx;
             const uint32_t nw = (lx * BITS + 31) / 32;
-            const uint32_t *xp = a.e_pk + (size_t)q * HW;
+            const uint32_t *xp = a.e_pk + (size_t)(tile.q0 + qi) * HW;
             uint32_t ham = 0;
 #pragma unroll
             for (int w = 0; w < HWMAX; ++w) {
                 if ((uint32_t)w < nw) {
-                    const uint32_t xw = xp[w];
+                    const uint32_t xw = QREG ? rl(qw[QREG ? w : 0], qi) : xp[w];
                     ham += BITS == 2 ? idb_mismatch2(xw, tw[w]) : idb_mismatch8(xw, tw[w]);
                 }
             }
-            // a negative fuse-pass threshold fails every non-empty pair
-            const bool hit = valid && ((int32_t)ham <= t || (lx == 0 && t < 0));
+            // count pass: the threshold is compared only after a mismatch
+            // (PSDBSCAN.hcu:623-636), fuse pass after every letter (:709-712);
+            // the per-length thresholds already encode both, and an empty row
+            // passes either way
+            const int32_t t = t_of(qt2, FUSE);
+            const bool hit = valid && ((int32_t)ham <= t || lx == 0);
             const unsigned long long hm = __ballot(hit);
             if (!hm) continue;
             if (!FUSE) {
                 hits += __popcll(hm);
-                if (hit) acc_y += (int32_t)a.e_cp[q];
+                if (hit) acc_y += (int32_t)rl(qe.cp, qi);
                 const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
                 if (lane == 0) atomicAdd(a.cnt + x, s);
-                // the fuse pass compares after every letter (PSDBSCAN.hcu:709-712):
-                // with a negative threshold no non-empty pair is an edge there
-                const int32_t tf = (int32_t)a.tlut_f[lx];
+                const int32_t tf = t_of(qt2, true);
                 stage_edges(a, est, hit && ((int32_t)ham <= tf || lx == 0), x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
@@ -471,7 +537,10 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     if (!FUSE) flush_edges(a, est, lane);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
-        if (!FUSE) atomicAdd(a.work + 1, hits);
+        if (!FUSE) {
+            atomicAdd(a.work + 1, hits);
+            atomicAdd(a.work + 3, gletters);
+        }
     }
 }
 
@@ -488,24 +557,35 @@ __global__ void k_union_edges(const uint2 *edges, unsigned long long n, const in
     }
 }
 
-uint32_t grid_for(uint32_t n_tiles, uint32_t wpb)
+// Blocks that fit on the device at once for this kernel and shape.
+uint32_t resident_blocks(const void *kernel, uint32_t block, size_t shm)
 {
-    // enough waves to fill 256 CUs several times over; each wave then walks
-    // tiles with a stride of the total wave count
-    const uint32_t max_blocks = 256u * 32u / wpb;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, shm) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    (void)hipGetLastError();
+    return (uint32_t)per_cu * (uint32_t)cus;
+}
+
+uint32_t grid_for(uint32_t n_tiles, uint32_t wpb, uint32_t resident)
+{
     uint32_t b = (n_tiles + wpb - 1) / wpb;
-    if (b > max_blocks) b = max_blocks;
+    if (b > resident) b = resident;
     return b ? b : 1u;
 }
 
 template <int W, bool FUSE, bool FILTER>
-hipError_t launch_myers_w(const idb_pair_args &a, dim3 grid, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
+hipError_t launch_myers_w(const idb_pair_args &a, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
 {
+    const void *k = (const void *)k_pairs_myers<W, FUSE, FILTER>;
     if (shm > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_pairs_myers<W, FUSE, FILTER>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
+    const dim3 grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm)));
     hipLaunchKernelGGL((k_pairs_myers<W, FUSE, FILTER>), grid, block, shm, s, a, wpb);
     return hipGetLastError();
 }
@@ -520,35 +600,42 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
     if (wpb > 4) wpb = 4;
     if (wpb == 0) wpb = per_wave <= limit ? 1u : 0u;
     if (wpb == 0) return hipErrorInvalidValue;
-    const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
+    const dim3 block(64 * wpb);
     const size_t shm = per_wave * wpb;
-    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, true>(a, grid, block, shm, wpb, s);
+    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, true>(a, block, shm, wpb, s);
     switch (W) {
-    case 1: return launch_myers_w<1, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 2: return launch_myers_w<2, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 3: return launch_myers_w<3, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 4: return launch_myers_w<4, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 5: return launch_myers_w<5, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 6: return launch_myers_w<6, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 7: return launch_myers_w<7, FUSE, false>(a, grid, block, shm, wpb, s);
-    case 8: return launch_myers_w<8, FUSE, false>(a, grid, block, shm, wpb, s);
+    case 1: return launch_myers_w<1, FUSE, false>(a, block, shm, wpb, s);
+    case 2: return launch_myers_w<2, FUSE, false>(a, block, shm, wpb, s);
+    case 3: return launch_myers_w<3, FUSE, false>(a, block, shm, wpb, s);
+    case 4: return launch_myers_w<4, FUSE, false>(a, block, shm, wpb, s);
+    case 5: return launch_myers_w<5, FUSE, false>(a, block, shm, wpb, s);
+    case 6: return launch_myers_w<6, FUSE, false>(a, block, shm, wpb, s);
+    case 7: return launch_myers_w<7, FUSE, false>(a, block, shm, wpb, s);
+    case 8: return launch_myers_w<8, FUSE, false>(a, block, shm, wpb, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int HWMAX, int BITS, bool FUSE>
+hipError_t launch_hamming_hw(const idb_pair_args &a, hipStream_t s)
+{
+    const uint32_t wpb = 4;
+    const void *k = (const void *)k_pairs_hamming<HWMAX, BITS, FUSE>;
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, 64 * wpb, 0)));
+    hipLaunchKernelGGL((k_pairs_hamming<HWMAX, BITS, FUSE>), grid, block, 0, s, a, wpb);
+    return hipGetLastError();
 }
 
 template <int BITS, bool FUSE>
 hipError_t launch_hamming_bits(const idb_pair_args &a, hipStream_t s)
 {
-    const uint32_t wpb = 4;
-    const dim3 grid(grid_for(a.n_tiles, wpb)), block(64 * wpb);
-    if (a.HW <= 2) hipLaunchKernelGGL((k_pairs_hamming<2, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else if (a.HW <= 4) hipLaunchKernelGGL((k_pairs_hamming<4, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else if (a.HW <= 8) hipLaunchKernelGGL((k_pairs_hamming<8, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else if (a.HW <= 16) hipLaunchKernelGGL((k_pairs_hamming<16, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else if (a.HW <= 32) hipLaunchKernelGGL((k_pairs_hamming<32, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else if (a.HW <= 64) hipLaunchKernelGGL((k_pairs_hamming<64, BITS, FUSE>), grid, block, 0, s, a, wpb);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    if (a.HW <= 2) return launch_hamming_hw<2, BITS, FUSE>(a, s);
+    if (a.HW <= 4) return launch_hamming_hw<4, BITS, FUSE>(a, s);
+    if (a.HW <= 8) return launch_hamming_hw<8, BITS, FUSE>(a, s);
+    if (a.HW <= 16) return launch_hamming_hw<16, BITS, FUSE>(a, s);
+    if (a.HW <= 32) return launch_hamming_hw<32, BITS, FUSE>(a, s);
+    if (a.HW <= 64) return launch_hamming_hw<64, BITS, FUSE>(a, s);
+    return hipErrorInvalidValue;
 }
 
 template <bool FUSE>
