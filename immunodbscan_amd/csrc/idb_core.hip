@@ -495,6 +495,23 @@ __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, cons
     }
 }
 
+// Sum over the block, then one atomic per block: a per-thread atomic on one
+// address serialises.
+__device__ __forceinline__ void block_add(uint32_t v, uint32_t *dst)
+{
+    __shared__ uint32_t part[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < (blockDim.x + 63) / 64; ++k) t += part[k];
+        if (t) atomicAdd(dst, t);
+    }
+}
+
 __global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int32_t *core, uint32_t *small)
 {
     uint32_t local = 0;
@@ -503,7 +520,7 @@ __global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int
         core[i] = c;
         local += (uint32_t)c;
     }
-    if (local) atomicAdd(small + 9, local);
+    block_add(local, small + 9);
 }
 
 __device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t x)
@@ -568,7 +585,7 @@ __global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_
         cop_out[i] = copies[i];
         if (cnt_out) cnt_out[i] = (uint32_t)cnt[i];
     }
-    if (noise) atomicAdd(small + 10, noise);
+    block_add(noise, small + 10);
 }
 
 // First unique position of every repertoire (binary search over uniques).

@@ -544,6 +544,46 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     }
 }
 
+// Union over the recorded pairs.  Parents are read through the caches and
+// never rewritten except by the linking exchange: a stale parent only points
+// to an earlier ancestor (parents only decrease), which ends in a failed
+// exchange that returns the current value, so every retry still makes
+// progress; without path-halving stores, hot roots see no write traffic
+// besides the links themselves.  Roots are compressed afterwards.
+__device__ __forceinline__ uint32_t ld_cached(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t find_cached(const uint32_t *par, uint32_t x)
+{
+    uint32_t p = ld_cached(par + x);
+    while (p != x) {
+        x = p;
+        p = ld_cached(par + x);
+    }
+    return x;
+}
+
+__device__ void union_cached(uint32_t *par, uint32_t a, uint32_t b)
+{
+    a = find_cached(par, a);
+    b = find_cached(par, b);
+    while (a != b) {
+        if (a < b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        uint32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(par + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        a = find_cached(par, expected);
+        b = find_cached(par, b);
+    }
+}
+
 __global__ void k_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
                               uint32_t *bmin)
 {
@@ -551,7 +591,7 @@ __global__ void k_union_edges(const uint2 *edges, unsigned long long n, const in
          e += (unsigned long long)gridDim.x * blockDim.x) {
         const uint2 p = edges[e];
         const int32_t cx = core[p.x], cy = core[p.y];
-        if (cx && cy) uf_union(par, p.x, p.y);
+        if (cx && cy) union_cached(par, p.x, p.y);
         else if (cx) atomicMin(bmin + p.y, p.x);
         else if (cy) atomicMin(bmin + p.x, p.y);
     }
