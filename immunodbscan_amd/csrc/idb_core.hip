@@ -39,7 +39,7 @@ enum Slot {
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
-    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT,
+    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL,
     S_COUNT
 };
 
@@ -143,6 +143,48 @@ __device__ __forceinline__ uint64_t fin64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// Bytes k .. k+15 of a row of length L as one 16-byte value, zero past the
+// row's end.  Every load stays inside the row: a short tail is read as the
+// row's last 16 bytes and shifted down, rows under 16 bytes byte by byte.
+// Unaligned 16-byte global loads are legal on gfx950.
+__device__ __forceinline__ uint4 row16(const uint8_t *s, uint32_t L, uint32_t k)
+{
+    uint4 v;
+    if (k + 16 <= L) {
+        __builtin_memcpy(&v, s + k, 16);
+        return v;
+    }
+    const uint32_t r = L - k;  // 1 .. 15 bytes left
+    uint64_t lo = 0, hi = 0;
+    if (L >= 16) {
+        uint64_t t[2];
+        __builtin_memcpy(t, s + L - 16, 16);
+        const uint32_t sh = 8 * (16 - r);  // 8 .. 120 bits
+        if (sh >= 64) {
+            lo = t[1] >> (sh - 64);
+        } else {
+            lo = (t[0] >> sh) | (t[1] << (64 - sh));
+            hi = t[1] >> sh;
+        }
+    } else {
+        for (uint32_t b = 0; b < r; ++b) {
+            const uint64_t c = s[k + b];
+            if (b < 8) lo |= c << (8 * b); else hi |= c << (8 * (b - 8));
+        }
+    }
+    v.x = (uint32_t)lo;
+    v.y = (uint32_t)(lo >> 32);
+    v.z = (uint32_t)hi;
+    v.w = (uint32_t)(hi >> 32);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t byte_of(const uint4 &v, uint32_t b)
+{
+    const uint32_t w = b < 8 ? (b < 4 ? v.x : v.y) : (b < 12 ? v.z : v.w);
+    return (w >> (8 * (b & 3))) & 0xFFu;
+}
+
 __global__ void k_fill_rid(const uint32_t *rstart, uint32_t n_reps, uint32_t n, uint32_t *rid)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -156,7 +198,8 @@ __global__ void k_fill_rid(const uint32_t *rstart, uint32_t n_reps, uint32_t n, 
     }
 }
 
-// Row scan: content hash (R3 key), letter set, longest row.
+// Row scan: content hash (R3 key), letter set, longest row.  Letters below
+// 128 are collected in per-thread masks and merged once per wave.
 __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
                           const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t n, uint64_t hmask,
                           uint64_t *key, uint32_t *idx, uint32_t *small)
@@ -166,18 +209,29 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
     if (threadIdx.x < 8) sym[threadIdx.x] = 0;
     if (threadIdx.x == 0) smax = 0;
     __syncthreads();
+    uint64_t m_lo = 0, m_hi = 0;
+    uint32_t lmax = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t L = len[i];
         const uint8_t *s = seq + off[i];
         uint64_t h = mix64(0x1234567ull, ((uint64_t)L << 32) | (rid ? rid[i] : 0u));
         uint64_t acc = 0;
-        for (uint32_t k = 0; k < L; ++k) {
-            const uint32_t b = s[k];
-            atomicOr(&sym[b >> 5], 1u << (b & 31));
-            acc = (acc << 8) | b;
-            if ((k & 7) == 7) {
-                h = mix64(h, acc);
-                acc = 0;
+        for (uint32_t k0 = 0; k0 < L; k0 += 16) {
+            const uint4 v = row16(s, L, k0);
+#pragma unroll
+            for (uint32_t b = 0; b < 16; ++b) {
+                const uint32_t k = k0 + b;
+                if (k < L) {
+                    const uint32_t c = byte_of(v, b);
+                    if (c < 64) m_lo |= 1ull << c;
+                    else if (c < 128) m_hi |= 1ull << (c - 64);
+                    else atomicOr(&sym[c >> 5], 1u << (c & 31));
+                    acc = (acc << 8) | c;
+                    if ((k & 7) == 7) {
+                        h = mix64(h, acc);
+                        acc = 0;
+                    }
+                }
             }
         }
         h = mix64(h, acc);
@@ -185,7 +239,25 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
         for (uint32_t w = 0; w < jw; ++w) h = mix64(h, 0x100000000ull | (uint32_t)jb[(size_t)i * jw + w]);
         key[i] = fin64(h) & hmask;
         idx[i] = i;
-        atomicMax(&smax, L);
+        lmax = L > lmax ? L : lmax;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        m_lo |= __shfl_xor(m_lo, o, 64);
+        m_hi |= __shfl_xor(m_hi, o, 64);
+        const uint32_t t = __shfl_xor(lmax, o, 64);
+        lmax = t > lmax ? t : lmax;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (m_lo) {
+            atomicOr(&sym[0], (uint32_t)m_lo);
+            atomicOr(&sym[1], (uint32_t)(m_lo >> 32));
+        }
+        if (m_hi) {
+            atomicOr(&sym[2], (uint32_t)m_hi);
+            atomicOr(&sym[3], (uint32_t)(m_hi >> 32));
+        }
+        atomicMax(&smax, lmax);
     }
     __syncthreads();
     if (threadIdx.x < 8 && sym[threadIdx.x]) atomicOr(&small[threadIdx.x], sym[threadIdx.x]);
@@ -210,8 +282,11 @@ __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32
     for (uint32_t w = 0; w < jw; ++w)
         if (jb[(size_t)a * jw + w] != jb[(size_t)b * jw + w]) return false;
     const uint8_t *sa = seq + off[a], *sb = seq + off[b];
-    for (uint32_t k = 0; k < len[a]; ++k)
-        if (sa[k] != sb[k]) return false;
+    const uint32_t L = len[a];
+    for (uint32_t k = 0; k < L; k += 16) {
+        const uint4 x = row16(sa, L, k), y = row16(sb, L, k);
+        if (x.x != y.x || x.y != y.y || x.z != y.z || x.w != y.w) return false;
+    }
     return true;
 }
 
@@ -325,16 +400,20 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const uint64_t *off, const uint32_t *copies, const int32_t *vb, uint32_t vw,
                                const int32_t *jb, uint32_t jw, uint32_t single, uint32_t gj, uint32_t gvj,
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
-                               const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint16_t *elen,
-                               idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
+                               const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len,
+                               uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
 {
+    __shared__ uint8_t cm[256];
+    for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) cm[b] = cmap[b];
+    __syncthreads();
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const uint64_t k = ekey[e];
         const uint32_t u = erow[e];
         const uint32_t L = (uint32_t)(k & 0xFFFFu);
         const uint64_t bucket = k >> 16;
         elen[e] = (uint16_t)L;
-        bflag[e] = (e == 0) || ((ekey[e - 1] >> 16) != bucket);
+        // a group is a bucket, or in Hamming mode a bucket's run of one length
+        bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> 16) != bucket);
         idb_entry r;
         r.row = u;
         r.cp = copies[u];
@@ -354,10 +433,15 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         const uint8_t *s = seq + off[u];
         uint32_t p1 = 0, p2 = 0;
         if (pres)
-            for (uint32_t pos = 0; pos < L; ++pos) {
-                const uint32_t bit = 1u << cmap[s[pos]];
-                p2 |= p1 & bit;
-                p1 |= bit;
+            for (uint32_t k0 = 0; k0 < L; k0 += 16) {
+                const uint4 v = row16(s, L, k0);
+#pragma unroll
+                for (uint32_t b = 0; b < 16; ++b)
+                    if (k0 + b < L) {
+                        const uint32_t bit = 1u << cm[byte_of(v, b)];
+                        p2 |= p1 & bit;
+                        p1 |= bit;
+                    }
             }
         r.pres1 = p1;
         r.pres2 = p2;
@@ -365,26 +449,41 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         ent[e] = r;
         if (ecodes) {
             // letters past the row's end carry the sentinel code `pad`
-            uint32_t *dst = ecodes + (size_t)e * (S / 4);
-            for (uint32_t w = 0; w < S / 4; ++w) {
-                uint32_t word = 0;
-                for (uint32_t b = 0; b < 4; ++b) {
-                    const uint32_t pos = w * 4 + b;
-                    word |= (pos < L ? (uint32_t)cmap[s[pos]] : pad) << (8 * b);
+            uint4 *dst = (uint4 *)(ecodes + (size_t)e * (S / 4));
+            for (uint32_t k0 = 0; k0 < S; k0 += 16) {
+                const uint4 v = k0 < L ? row16(s, L, k0) : make_uint4(0, 0, 0, 0);
+                uint32_t w[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (uint32_t b = 0; b < 4; ++b) {
+                        const uint32_t pos = k0 + q * 4 + b;
+                        word |= (pos < L ? (uint32_t)cm[byte_of(v, q * 4 + b)] : pad) << (8 * b);
+                    }
+                    w[q] = word;
                 }
-                dst[w] = word;
+                dst[k0 / 16] = make_uint4(w[0], w[1], w[2], w[3]);
             }
         }
         if (epk) {
+            // 16 letters per step: one 2-bit word or four 8-bit words
             uint32_t *dst = epk + (size_t)e * HW;
             const uint32_t per = 32 / bits;
-            for (uint32_t w = 0; w < HW; ++w) {
-                uint32_t word = 0;
-                for (uint32_t b = 0; b < per; ++b) {
-                    const uint32_t pos = w * per + b;
-                    if (pos < L) word |= (uint32_t)cmap[s[pos]] << (bits * b);
-                }
-                dst[w] = word;
+            for (uint32_t k0 = 0; k0 < HW * per; k0 += 16) {
+                const uint4 v = k0 < L ? row16(s, L, k0) : make_uint4(0, 0, 0, 0);
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (uint32_t b = 0; b < 16; ++b)
+                    if (k0 + b < L) {
+                        const uint32_t code = cm[byte_of(v, b)];
+                        if (bits == 2) w[0] |= code << (2 * b);
+                        else w[b >> 2] |= code << (8 * (b & 3));
+                    }
+                const uint32_t w0 = k0 / per, nw = bits == 2 ? 1u : 4u;
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (q < nw && w0 + q < HW) dst[w0 + q] = w[q];
             }
         }
     }
@@ -619,10 +718,12 @@ __global__ void k_uniques_out(const uint32_t *uniq, const uint32_t *num, const u
     }
 }
 
+// Grid-stride kernels run at most 8 blocks per CU: enough to fill the chip,
+// few enough that per-block counter atomics stay cheap.
 uint32_t blocks_for(uint64_t n, uint32_t bs = 256)
 {
     uint64_t b = (n + bs - 1) / bs;
-    if (b > 65535u) b = 65535u;
+    if (b > 2048u) b = 2048u;
     return b ? (uint32_t)b : 1u;
 }
 
@@ -873,8 +974,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // rare (protein alphabets); with 4 letters nearly every pair passes.
         const bool filt = !ham && W == 1 && K >= 8 && K <= 31;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
-                           jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u, elen, ent,
-                           ecodes, epk, bflag);
+                           jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
+                           ham ? 1u : 0u, elen, ent, ecodes, epk, bflag);
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
         uint32_t *qbcnt = buf<uint32_t>(c, S_QBCNT, (size_t)E + 1);
@@ -914,11 +1015,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.work = work;
         // Neighbour pairs found by the count pass are kept for the fuse step;
         // if they overflow the list, the fuse pass re-evaluates the tiles.
-        edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 20, 8ull * E);
+        // Waves own chunks of 256 pairs (64 for small requested capacities).
+        edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 8ull * E);
         edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 29);
+        a.chunk_log2 = edge_cap >= (1ull << 20) ? 8u : 6u;
+        a.max_chunks = (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
+        edge_cap = (unsigned long long)a.max_chunks << a.chunk_log2;
         a.edges = buf<uint2>(c, S_EDGES, edge_cap);
-        a.edge_cap = edge_cap;
-        a.n_edges = work + 4;
+        a.chunk_fill = buf<uint32_t>(c, S_FILL, a.max_chunks);
+        a.n_chunks = small + 12;
         check(c, hipEventRecord(c->kev[0], s), "event record");
         check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
         check(c, hipEventRecord(c->kev[1], s), "event record");
@@ -929,14 +1034,14 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
 
     // --- Grow clusters: union over the recorded neighbour pairs, or a fuse
     // pass over the tiles when the pair list overflowed
-    unsigned long long n_edges = 0;
     bool from_edges = false;
     if (E > 0) {
-        readback(c, &n_edges, work + 4, sizeof(n_edges));
-        from_edges = n_edges <= edge_cap;
+        uint32_t n_chunks = 0;
+        readback(c, &n_chunks, small + 12, sizeof(n_chunks));
+        from_edges = n_chunks <= a.max_chunks;
         check(c, hipEventRecord(c->kev[2], s), "event record");
         if (from_edges) {
-            check(c, idb_launch_union_edges(a.edges, n_edges, core, par, bmin, s), "union kernel launch");
+            check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
         } else {
             a.core = core;
             a.par = par;

@@ -14,7 +14,11 @@
 //             (stride S bytes, S a multiple of 16) or, for Hamming mode,
 //             packed words (stride HW words).
 //   tiles:    (query block, target block) pairs of <= 64 entries each inside a
-//             bucket whose length band can hold a neighbour pair.
+//             group (a bucket; in Hamming mode a bucket's run of one length)
+//             whose length band can hold a neighbour pair.
+//   edges:    neighbour pairs of the count pass in chunks of 2^chunk_log2
+//             pairs; a wave fills one chunk at a time, chunk_fill[c] says how
+//             many pairs chunk c holds.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -66,16 +70,20 @@ struct idb_pair_args {
     uint32_t *bmin;               // fuse pass: border claims
     unsigned long long *work;     // [0] gated pairs, [1] neighbour pairs, [2] fuse pairs, [3] gated letters,
                                   // [4] edges, [5] Myers pairs, [6] Myers letters
-    uint2 *edges;                 // count pass: neighbour pairs (x, y), may be null
-    unsigned long long edge_cap;
-    unsigned long long *n_edges;
+    uint2 *edges;                 // count pass: neighbour pairs (x, y) in chunks, may be null
+    uint32_t *chunk_fill;         // pairs written into each chunk
+    uint32_t *n_chunks;           // chunks handed out (counts past max_chunks on overflow)
+    uint32_t max_chunks;          // capacity in chunks
+    uint32_t chunk_log2;          // pairs per chunk = 1 << chunk_log2 (>= IDB_TILE)
+    uint32_t fuse_if_overflow;    // fuse pass: run only when the pair list overflowed
 };
 
 // LDS bytes one wave of the Myers kernels needs: Peq columns, the edge
-// staging slots and the Myers queue (2 * IDB_TILE 16-bit entries).
+// staging slots, the Myers queue (2 * IDB_TILE 16-bit entries) and the
+// per-tile count accumulators of queries and targets.
 __host__ __device__ inline size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
 {
-    return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE) * sizeof(uint32_t);
+    return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t);
 }
 // The device's LDS limit per block (opt-in value where the device has one).
 size_t idb_lds_limit();
@@ -85,5 +93,5 @@ hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s);
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s);
-hipError_t idb_launch_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
-                                  uint32_t *bmin, hipStream_t s);
+hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
+                                  hipStream_t s);

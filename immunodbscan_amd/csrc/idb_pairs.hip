@@ -36,13 +36,6 @@ namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 {
 #pragma unroll
@@ -132,26 +125,39 @@ __device__ __forceinline__ bool gate_pair(const idb_pair_args &a, bool chk_x, bo
 }
 
 // Neighbour pairs are staged per wave in LDS (IDB_TILE entries) and appended
-// to the global edge list one chunk at a time, so the list's counter sees one
-// atomic per chunk instead of one per hit.  Past the capacity the 64-bit
-// counter keeps growing, so the host sees the overflow and runs the fuse pass
-// over the tiles instead.
+// to a chunk of the edge list that the wave owns; a new chunk costs one
+// atomic on the chunk counter, so that counter sees one atomic per chunk
+// instead of one per hit or per stage.  Past the capacity the counter keeps
+// growing and the pairs are dropped, so the overflow is visible and the fuse
+// pass over the tiles runs instead.
 struct EdgeStage {
-    uint2 *buf;   // this wave's IDB_TILE staging slots
-    uint32_t n;   // wave-uniform fill level
+    uint2 *buf;       // this wave's IDB_TILE staging slots
+    uint32_t n;       // staged pairs (wave-uniform)
+    uint32_t chunk;   // the wave's current chunk, IDB_NONE before the first
+    uint32_t fill;    // pairs written into it
+    uint32_t total;   // pairs this wave produced
 };
+
+__device__ __forceinline__ void close_chunk(const idb_pair_args &a, const EdgeStage &st, uint32_t lane)
+{
+    if (st.chunk != IDB_NONE && st.chunk < a.max_chunks && lane == 0) a.chunk_fill[st.chunk] = st.fill;
+}
 
 __device__ __forceinline__ void flush_edges(const idb_pair_args &a, EdgeStage &st, uint32_t lane)
 {
     if (st.n == 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(a.n_edges, (unsigned long long)st.n);
-    base = __shfl(base, 0, 64);
-    if (lane < st.n) {
-        const unsigned long long r = base + lane;
-        if (r < a.edge_cap) a.edges[r] = st.buf[lane];
+    if (st.chunk == IDB_NONE || st.fill + st.n > (1u << a.chunk_log2)) {
+        close_chunk(a, st, lane);
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(a.n_chunks, 1u);
+        st.chunk = __shfl(c, 0, 64);
+        st.fill = 0;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    if (st.chunk < a.max_chunks && lane < st.n)
+        a.edges[((size_t)st.chunk << a.chunk_log2) + st.fill + lane] = st.buf[lane];
+    st.fill += st.n;
+    st.total += st.n;
     st.n = 0;
 }
 
@@ -167,6 +173,22 @@ __device__ __forceinline__ void stage_edges(const idb_pair_args &a, EdgeStage &s
         st.buf[st.n + below] = make_uint2(x, y);
     }
     st.n += cnt;
+}
+
+// End of a count-pass wave: last stage, last chunk's fill, pair total.
+__device__ __forceinline__ void finish_edges(const idb_pair_args &a, EdgeStage &st, uint32_t lane)
+{
+    if (!a.edges) return;
+    flush_edges(a, st, lane);
+    close_chunk(a, st, lane);
+    if (lane == 0 && st.total && a.work) atomicAdd(a.work + 4, (unsigned long long)st.total);
+}
+
+// Fuse pass launched behind the union step without a host decision: it runs
+// only when the pair list overflowed.
+__device__ __forceinline__ bool fuse_skipped(const idb_pair_args &a)
+{
+    return a.fuse_if_overflow && *(volatile const uint32_t *)a.n_chunks <= a.max_chunks;
 }
 
 // The grid holds only as many waves as can be resident at once (no partly
@@ -206,14 +228,16 @@ __device__ __forceinline__ idb_entry load_entry(const idb_entry *e)
 }
 
 // Hit bookkeeping of the batched Myers path (one pair per lane).
+// Counts go to the tile's LDS accumulators of query qi and target tl.
 template <bool FUSE>
 __device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &est, bool hit, uint32_t x, uint32_t y,
-                                           uint32_t cx, uint32_t cy, int32_t core_x, int32_t core_y, uint32_t lane)
+                                           uint32_t cx, uint32_t cy, int32_t core_x, int32_t core_y, uint32_t lane,
+                                           int32_t *qacc, int32_t *tacc, uint32_t qi, uint32_t tl)
 {
     if (!FUSE) {
         if (hit) {
-            atomicAdd(a.cnt + x, (int32_t)cy);
-            atomicAdd(a.cnt + y, (int32_t)cx);
+            atomicAdd(qacc + qi, (int32_t)cy);
+            atomicAdd(tacc + tl, (int32_t)cx);
         }
         stage_edges(a, est, hit, x, y, lane);
     } else if (hit) {
@@ -230,7 +254,8 @@ template <int W, bool FUSE>
 __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
                                             const uint16_t *queue, uint32_t n, uint32_t lane, const idb_entry &qe,
                                             const idb_entry &te, int32_t core_y, int32_t q_core, EdgeStage &est,
-                                            unsigned long long &hits, unsigned long long &mlet)
+                                            int32_t *qacc, int32_t *tacc, unsigned long long &hits,
+                                            unsigned long long &mlet)
 {
     // Every cross-lane read runs with the whole wave active: a cross-lane
     // read from a lane that is switched off returns 0.
@@ -270,7 +295,7 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     const int32_t dist = tm ? st.score : (int32_t)lx;
     const bool hit = act && dist <= t;
     if (!FUSE) hits += __popcll(__ballot(hit));
-    record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane);
+    record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
 }
 
 // FILTER (alphabets of at most 31 letters): before Myers, a pair must pass the
@@ -284,8 +309,11 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t K = a.K;
     uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
-    EdgeStage est{(uint2 *)(peq + K * W * 64), 0u};
+    EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
     uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
+    int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
+    int32_t *tacc = qacc + IDB_TILE;
+    if (FUSE && fuse_skipped(a)) return;
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
@@ -331,6 +359,10 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
         int32_t acc_y = 0;
         uint32_t qlen = 0;  // queued pairs (FILTER)
+        if (!FUSE) {
+            qacc[lane] = 0;
+            tacc[lane] = 0;
+        }
 
         for (uint32_t qi = 0; qi < qn; ++qi) {
             const uint32_t x = rl(qe.row, qi);
@@ -370,8 +402,8 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 mpairs += __popcll(pm);
                 if (qlen >= IDB_TILE) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, hits,
-                                         mletters);
+                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, qacc,
+                                         tacc, hits, mletters);
                     const uint32_t rest = qlen - IDB_TILE;
                     const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -405,9 +437,10 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (!hm) continue;
             if (!FUSE) {
                 hits += __popcll(hm);
-                if (hit) acc_y += (int32_t)rl(qe.cp, qi);
-                const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
-                if (lane == 0) atomicAdd(a.cnt + x, s);
+                if (hit) {
+                    acc_y += (int32_t)rl(qe.cp, qi);
+                    atomicAdd(qacc + qi, (int32_t)cy);
+                }
                 stage_edges(a, est, hit, x, y, lane);
             } else if (hit) {
                 if (core_x && core_y) uf_union(a.par, x, y);
@@ -417,11 +450,19 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         if (FILTER && qlen) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, hits, mletters);
+            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
+                                 mletters);
         }
-        if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
+        if (!FUSE) {
+            // one atomic per row and tile
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            const int32_t qa = qacc[lane];
+            acc_y += tacc[lane];
+            if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
+            if (acc_y) atomicAdd(a.cnt + y, acc_y);
+        }
     }
-    if (!FUSE) flush_edges(a, est, lane);
+    if (!FUSE) finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
@@ -446,7 +487,10 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
     const uint32_t HW = a.HW;
     __shared__ uint2 stage[4][IDB_TILE];
-    EdgeStage est{stage[wib], 0u};
+    __shared__ int32_t qacc_s[4][IDB_TILE];
+    EdgeStage est{stage[wib], 0u, IDB_NONE, 0u, 0u};
+    int32_t *qacc = qacc_s[wib];
+    if (FUSE && fuse_skipped(a)) return;
     constexpr bool QREG = HWMAX <= 8;
     unsigned long long gated = 0, hits = 0, gletters = 0;
 
@@ -484,6 +528,7 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
             if (lane < qn) q_core = a.core[qe.row];
         }
         int32_t acc_y = 0;
+        if (!FUSE) qacc[lane] = 0;
         for (uint32_t qi = 0; qi < qn; ++qi) {
             const uint32_t x = rl(qe.row, qi);
             const uint32_t lc = rl(qe.len_chk, qi);
@@ -521,9 +566,10 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
             if (!hm) continue;
             if (!FUSE) {
                 hits += __popcll(hm);
-                if (hit) acc_y += (int32_t)rl(qe.cp, qi);
-                const int32_t s = wave_sum(hit ? (int32_t)cy : 0);
-                if (lane == 0) atomicAdd(a.cnt + x, s);
+                if (hit) {
+                    acc_y += (int32_t)rl(qe.cp, qi);
+                    atomicAdd(qacc + qi, (int32_t)cy);
+                }
                 const int32_t tf = t_of(qt2, true);
                 stage_edges(a, est, hit && ((int32_t)ham <= tf || lx == 0), x, y, lane);
             } else if (hit) {
@@ -532,9 +578,14 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
                 else atomicMin(a.bmin + x, y);
             }
         }
-        if (!FUSE && acc_y) atomicAdd(a.cnt + y, acc_y);
+        if (!FUSE) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            const int32_t qa = qacc[lane];
+            if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
+            if (acc_y) atomicAdd(a.cnt + y, acc_y);
+        }
     }
-    if (!FUSE) flush_edges(a, est, lane);
+    if (!FUSE) finish_edges(a, est, lane);
     if (lane == 0 && a.work) {
         atomicAdd(a.work + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
@@ -584,11 +635,18 @@ __device__ void union_cached(uint32_t *par, uint32_t a, uint32_t b)
     }
 }
 
-__global__ void k_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
-                              uint32_t *bmin)
+// The chunk count is read on the device, so the launch does not wait for the
+// count pass on the host; after an overflow the kernel does nothing.
+__global__ void k_union_edges(const uint2 *edges, const uint32_t *fill, const uint32_t *n_chunks, uint32_t max_chunks,
+                              uint32_t chunk_log2, const int32_t *core, uint32_t *par, uint32_t *bmin)
 {
+    const uint32_t nc = *n_chunks;
+    if (nc > max_chunks) return;
+    const unsigned long long n = (unsigned long long)nc << chunk_log2;
+    const uint32_t cmask = (1u << chunk_log2) - 1u;
     for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
          e += (unsigned long long)gridDim.x * blockDim.x) {
+        if (((uint32_t)e & cmask) >= fill[e >> chunk_log2]) continue;
         const uint2 p = edges[e];
         const int32_t cx = core[p.x], cy = core[p.y];
         if (cx && cy) union_cached(par, p.x, p.y);
@@ -706,12 +764,12 @@ hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<false>(a, s); }
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<true>(a, s); }
 
-hipError_t idb_launch_union_edges(const uint2 *edges, unsigned long long n, const int32_t *core, uint32_t *par,
-                                  uint32_t *bmin, hipStream_t s)
+hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
+                                  hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
-    unsigned long long blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_union_edges, dim3((uint32_t)blocks), dim3(256), 0, s, edges, n, core, par, bmin);
+    if (!a.edges || a.max_chunks == 0) return hipSuccess;
+    const uint32_t blocks = resident_blocks((const void *)k_union_edges, 256, 0);
+    hipLaunchKernelGGL(k_union_edges, dim3(blocks), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
+                       a.chunk_log2, core, par, bmin);
     return hipGetLastError();
 }
