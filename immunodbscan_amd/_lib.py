@@ -41,7 +41,21 @@ EXPORTED = [
     "idb_gene_bitsets",
     "idb_dbscan",
     "idb_dbscan_batch",
+    "idb_pair_distances",
+    "idb_neighbour_pairs",
+    "idb_encoder_create",
+    "idb_encoder_destroy",
+    "idb_encoder_last_error",
+    "idb_encoder_param_count",
+    "idb_encoder_get_params",
+    "idb_encoder_set_params",
+    "idb_encoder_embed",
+    "idb_encoder_loss_grad",
+    "idb_encoder_train",
+    "idb_dbscan_learned",
 ]
+
+IDB_ENC_PARAMS = 193984
 
 
 class IdbDataset(C.Structure):
@@ -116,6 +130,31 @@ class IdbStats(C.Structure):
         return d
 
 
+class IdbTrainParams(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_uint32),
+        ("batch_pairs", C.c_uint32),
+        ("lr", C.c_float),
+        ("margin", C.c_float),
+        ("seed", C.c_uint64),
+        ("log_every", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class IdbTrainStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint32),
+        ("first_loss", C.c_float),
+        ("last_loss", C.c_float),
+        ("mean_sampled_loss_last_epoch", C.c_float),
+        ("train_ms", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -136,6 +175,26 @@ def _load() -> C.CDLL:
     lib.idb_dbscan_batch.argtypes = [vp, C.POINTER(IdbDataset), vp, u32, C.POINTER(IdbParams),
                                      C.POINTER(IdbOutputs), vp, C.POINTER(IdbStats)]
     lib.idb_dbscan_batch.restype = C.c_int
+    DS, PR, OUT, ST = C.POINTER(IdbDataset), C.POINTER(IdbParams), C.POINTER(IdbOutputs), C.POINTER(IdbStats)
+    f32 = C.c_float
+    sigs = {
+        "idb_pair_distances": ([vp, DS, vp, u64, u32, vp], C.c_int),
+        "idb_neighbour_pairs": ([vp, DS, PR, vp, u64, C.POINTER(u64)], C.c_int),
+        "idb_encoder_create": ([C.c_int, vp, u64, C.POINTER(vp)], C.c_int),
+        "idb_encoder_destroy": ([vp], None),
+        "idb_encoder_last_error": ([vp], C.c_char_p),
+        "idb_encoder_param_count": ([], u32),
+        "idb_encoder_get_params": ([vp, vp], C.c_int),
+        "idb_encoder_set_params": ([vp, vp], C.c_int),
+        "idb_encoder_embed": ([vp, DS, vp], C.c_int),
+        "idb_encoder_loss_grad": ([vp, DS, vp, vp, u32, f32, C.POINTER(f32), vp], C.c_int),
+        "idb_encoder_train": ([vp, DS, vp, vp, u64, C.POINTER(IdbTrainParams), C.POINTER(IdbTrainStats)], C.c_int),
+        "idb_dbscan_learned": ([vp, vp, f32, DS, PR, OUT, ST], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
     return lib
 
 

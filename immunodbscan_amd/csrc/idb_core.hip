@@ -39,7 +39,8 @@ enum Slot {
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
-    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL,
+    S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
+    S_DIST,
     S_COUNT
 };
 
@@ -787,12 +788,23 @@ unsigned bits_for(uint64_t v)
 
 void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "event record"); }
 
+// Options beyond the reference's run: the learned candidate filter, and a
+// copy of the neighbour pairs for the caller.
+struct Extra {
+    idb_encoder *enc = nullptr;
+    float min_sim = 0.f;
+    uint32_t *pairs_out = nullptr;
+    uint64_t pairs_cap = 0;
+    uint64_t *n_pairs_out = nullptr;
+};
+
 // ---------------------------------------------------------------------------
 // The pipeline.  rstart (host) has n_reps + 1 row offsets; n_reps == 0 means a
 // single repertoire without repertoire ids.
 void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_reps, const idb_params *p,
-         idb_outputs *o, uint32_t *n_uniques_h, idb_stats *st)
+         idb_outputs *o, uint32_t *n_uniques_h, idb_stats *st, const Extra *x = nullptr)
 {
+    const bool learned = x && x->enc;
     const uint32_t N = d->n_rows;
     const uint32_t vw = d->v_width, jw = d->j_width;
     if (o->mem != IDB_MEM_HOST && o->mem != IDB_MEM_DEVICE) fail(c, IDB_ERR_ARG, "outputs.mem must be host or device");
@@ -866,6 +878,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
                                              std::to_string(maxlen) + " letters exceeds the LDS budget");
         S = std::max(16u, (maxlen + 15) / 16 * 16);
     }
+    if (learned && (ham || W != 1 || maxlen > IDB_ENC_P))
+        fail(c, IDB_ERR_UNSUPPORTED, "the learned filter needs Levenshtein mode and rows of at most 32 letters");
+    if (learned && idb_encoder_device(x->enc) != c->device)
+        fail(c, IDB_ERR_ARG, "the encoder lives on another device");
     sort_pairs_u64(c, key, key2, idx, idx2, N, 64);
     uint32_t *head0 = buf<uint32_t>(c, S_HEAD, N);
     uint32_t *rep = buf<uint32_t>(c, S_REP, N);
@@ -1024,8 +1040,17 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.edges = buf<uint2>(c, S_EDGES, edge_cap);
         a.chunk_fill = buf<uint32_t>(c, S_FILL, a.max_chunks);
         a.n_chunks = small + 12;
+        if (learned) {
+            // embeddings of every row, then proposals verified by Myers
+            uint16_t *zb = buf<uint16_t>(c, S_ZB, (size_t)N * 64);
+            if (idb_encoder_embed_rows(x->enc, seq, off, len, N, zb, s) != hipSuccess)
+                fail(c, IDB_ERR_HIP, std::string("encoder: ") + idb_encoder_error(x->enc));
+            a.zb = zb;
+            a.min_sim = x->min_sim;
+        }
         check(c, hipEventRecord(c->kev[0], s), "event record");
-        check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
+        if (learned) check(c, idb_launch_count_learned(a, s), "learned count kernel launch");
+        else check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
         check(c, hipEventRecord(c->kev[1], s), "event record");
     }
     int32_t *core = buf<int32_t>(c, S_CORE, N);
@@ -1039,6 +1064,29 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         uint32_t n_chunks = 0;
         readback(c, &n_chunks, small + 12, sizeof(n_chunks));
         from_edges = n_chunks <= a.max_chunks;
+        if (learned && !from_edges)
+            fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
+        if (x && x->pairs_out) {
+            if (!from_edges) fail(c, IDB_ERR_OOM, "neighbour pair list overflow: raise edge_capacity");
+            std::vector<uint32_t> fill(n_chunks);
+            std::vector<uint2> ed((size_t)n_chunks << a.chunk_log2);
+            if (n_chunks) {
+                check(c, hipMemcpyAsync(fill.data(), a.chunk_fill, sizeof(uint32_t) * n_chunks, hipMemcpyDeviceToHost, s),
+                      "pairs copy");
+                check(c, hipMemcpyAsync(ed.data(), a.edges, sizeof(uint2) * ed.size(), hipMemcpyDeviceToHost, s),
+                      "pairs copy");
+                check(c, hipStreamSynchronize(s), "pairs sync");
+            }
+            uint64_t k = 0;
+            for (uint32_t ch = 0; ch < n_chunks; ++ch)
+                for (uint32_t i = 0; i < fill[ch]; ++i, ++k) {
+                    if (k >= x->pairs_cap) continue;
+                    const uint2 e = ed[((size_t)ch << a.chunk_log2) + i];
+                    x->pairs_out[2 * k] = std::min(e.x, e.y);
+                    x->pairs_out[2 * k + 1] = std::max(e.x, e.y);
+                }
+            *x->n_pairs_out = k;
+        }
         check(c, hipEventRecord(c->kev[2], s), "event record");
         if (from_edges) {
             check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
@@ -1224,7 +1272,7 @@ int idb_gene_bitsets(uint32_t n, const uint8_t *calls, const uint64_t *call_off,
 }
 
 static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uint32_t nr, const idb_params *p,
-                       idb_outputs *o, uint32_t *nu, idb_stats *st)
+                       idb_outputs *o, uint32_t *nu, idb_stats *st, const Extra *x = nullptr)
 {
     if (!c) return IDB_ERR_ARG;
     if (!d || !p || !o) {
@@ -1234,7 +1282,7 @@ static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uin
     c->err.clear();
     try {
         check(c, hipSetDevice(c->device), "hipSetDevice");
-        run(c, d, rs, nr, p, o, nu, st);
+        run(c, d, rs, nr, p, o, nu, st, x);
     } catch (const Fail &f) {
         (void)hipGetLastError();
         (void)hipStreamSynchronize(c->stream);
@@ -1269,6 +1317,90 @@ int idb_dbscan_batch(idb_ctx *c, const idb_dataset *d, const uint32_t *rep_start
             return IDB_ERR_ARG;
         }
     return run_guarded(c, d, rep_start, n_reps, p, o, n_uniques, st);
+}
+
+int idb_dbscan_learned(idb_ctx *c, idb_encoder *enc, float min_similarity, const idb_dataset *d, const idb_params *p,
+                       idb_outputs *o, idb_stats *st)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!enc) {
+        c->err = "null encoder";
+        return IDB_ERR_ARG;
+    }
+    Extra x;
+    x.enc = enc;
+    x.min_sim = min_similarity;
+    return run_guarded(c, d, nullptr, 0, p, o, nullptr, st, &x);
+}
+
+int idb_neighbour_pairs(idb_ctx *c, const idb_dataset *d, const idb_params *p, uint32_t *pairs, uint64_t cap,
+                        uint64_t *n_pairs)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!d || !p || !n_pairs || (cap && !pairs)) {
+        c->err = "null argument";
+        return IDB_ERR_ARG;
+    }
+    const uint32_t N = d->n_rows;
+    std::vector<uint32_t> cid(N), cp(N), uq(N);
+    std::vector<int32_t> core(N);
+    idb_outputs o{IDB_MEM_HOST, cid.data(), core.data(), cp.data(), uq.data(), nullptr};
+    Extra x;
+    std::vector<uint32_t> sink(2);
+    x.pairs_out = pairs ? pairs : sink.data();
+    x.pairs_cap = pairs ? cap : 0;
+    x.n_pairs_out = n_pairs;
+    const int rc = run_guarded(c, d, nullptr, 0, p, &o, nullptr, nullptr, &x);
+    if (rc == IDB_OK && *n_pairs > cap) {
+        c->err = "more neighbour pairs than cap";
+        return IDB_ERR_OOM;
+    }
+    return rc;
+}
+
+int idb_pair_distances(idb_ctx *c, const idb_dataset *d, const uint32_t *pairs, uint64_t n_pairs, uint32_t use_hamming,
+                       uint32_t *dist)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!d || (n_pairs && (!pairs || !dist))) {
+        c->err = "null argument";
+        return IDB_ERR_ARG;
+    }
+    c->err.clear();
+    try {
+        check(c, hipSetDevice(c->device), "hipSetDevice");
+        hipStream_t s = c->stream;
+        const uint32_t N = d->n_rows;
+        for (uint64_t i = 0; i < 2 * n_pairs; ++i)
+            if (pairs[i] >= N) fail(c, IDB_ERR_ARG, "pair row out of range");
+        if (n_pairs == 0) return IDB_OK;
+        std::vector<uint32_t> lh;
+        const uint32_t *len_h = d->len;
+        if (d->mem == IDB_MEM_DEVICE) {
+            lh.resize(N);
+            check(c, hipMemcpy(lh.data(), d->len, sizeof(uint32_t) * N, hipMemcpyDeviceToHost), "length copy");
+            len_h = lh.data();
+        }
+        uint32_t maxlen = 0;
+        for (uint64_t i = 0; i < 2 * n_pairs; ++i) maxlen = std::max(maxlen, len_h[pairs[i]]);
+        const uint32_t W = std::max(1u, (maxlen + 31) / 32);
+        if (!use_hamming && W > IDB_MAX_W) fail(c, IDB_ERR_UNSUPPORTED, "rows longer than 256 letters");
+        const uint8_t *seq = stage_in<uint8_t>(c, S_SEQ, d->seq, d->seq_bytes, d->mem);
+        const uint64_t *off = stage_in<uint64_t>(c, S_OFF, d->off, N, d->mem);
+        const uint32_t *len = stage_in<uint32_t>(c, S_LEN, d->len, N, d->mem);
+        if (!seq) seq = buf<uint8_t>(c, S_SEQ, 4);
+        uint2 *pd = buf<uint2>(c, S_PAIRS, n_pairs);
+        uint32_t *dd = buf<uint32_t>(c, S_DIST, n_pairs);
+        check(c, hipMemcpyAsync(pd, pairs, sizeof(uint2) * n_pairs, hipMemcpyHostToDevice, s), "pairs copy");
+        check(c, idb_launch_pair_distances(seq, off, len, pd, n_pairs, W, use_hamming, dd, s), "distance launch");
+        check(c, hipMemcpyAsync(dist, dd, sizeof(uint32_t) * n_pairs, hipMemcpyDeviceToHost, s), "distance copy");
+        check(c, hipStreamSynchronize(s), "sync");
+    } catch (const Fail &f) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(c->stream);
+        return f.code;
+    }
+    return IDB_OK;
 }
 
 }  // extern "C"

@@ -23,11 +23,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "idb.h"
+
 #define IDB_NONE 0xFFFFFFFFu
 #define IDB_TILE 64               // entries per query / target block (one wave)
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
 #define IDB_MAX_HW 64             // Hamming packed words per row
 #define IDB_MAX_LEN 65535u
+#define IDB_ENC_P 32              // letters per row the encoder reads
 
 struct idb_tile {
     uint32_t q0;                  // first query entry
@@ -76,6 +79,8 @@ struct idb_pair_args {
     uint32_t max_chunks;          // capacity in chunks
     uint32_t chunk_log2;          // pairs per chunk = 1 << chunk_log2 (>= IDB_TILE)
     uint32_t fuse_if_overflow;    // fuse pass: run only when the pair list overflowed
+    const uint16_t *zb;           // learned filter: bfloat16 embedding (64) per row
+    float min_sim;                // learned filter: proposal threshold on z_x . z_y
 };
 
 // LDS bytes one wave of the Myers kernels needs: Peq columns, the edge
@@ -93,5 +98,15 @@ hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s);
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s);
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s);
+hipError_t idb_launch_count_learned(const idb_pair_args &a, hipStream_t s);
+hipError_t idb_launch_pair_distances(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const uint2 *pairs,
+                                     unsigned long long n, uint32_t W, uint32_t ham, uint32_t *dist, hipStream_t s);
 hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
                                   hipStream_t s);
+
+// Encoder (idb_learned.hip): bfloat16 embeddings of rows [0, n) on stream s.
+struct idb_encoder;
+hipError_t idb_encoder_embed_rows(idb_encoder *e, const uint8_t *seq, const uint64_t *off, const uint32_t *len,
+                                  uint32_t n, uint16_t *zb, hipStream_t s);
+const char *idb_encoder_error(const idb_encoder *e);
+int idb_encoder_device(const idb_encoder *e);

@@ -475,6 +475,234 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     }
 }
 
+// Learned filter (rows of at most 32 letters): the tile's 64 x 64 embedding
+// dot products come from 16 MFMAs (v_mfma_f32_32x32x16_bf16, 2 x 2 output
+// tiles, 4 k-steps over the 64 dimensions); a pair is proposed when its dot
+// product reaches min_sim and it passes the same gate as the exact pass (and,
+// for alphabets of 8-31 letters, the exact absent-letter bound), and every
+// proposal is verified by Myers exactly as there.  Lane l of output tile
+// (i, j) holds target j * 32 + l % 32 and queries i * 32 + 8 (r / 4) +
+// 4 (l / 32) + r % 4, r = 0 .. 15; query records are staged in LDS for the
+// gate, target fields come from the owning lane once per tile.  The target
+// Peq columns are built only for tiles that have a proposal.
+typedef __bf16 idb_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float idb_f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ idb_bf16x8 load_z8(const uint16_t *zb, uint32_t row, uint32_t k, bool valid)
+{
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (valid) v = *(const uint4 *)(zb + (size_t)row * 64 + k);
+    return __builtin_bit_cast(idb_bf16x8, v);
+}
+
+__global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t wpb)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t K = a.K;
+    constexpr int W = 1;
+    // per wave: Peq, edge stage, queue, count accumulators, query records
+    uint32_t *peq = lds + (size_t)wib * ((idb_myers_lds_per_wave(K, W) + IDB_TILE * sizeof(idb_entry)) / sizeof(uint32_t));
+    EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
+    uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
+    int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
+    int32_t *tacc = qacc + IDB_TILE;
+    idb_entry *qrec = (idb_entry *)(tacc + IDB_TILE);
+    const uint32_t total_waves = gridDim.x * wpb;
+    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
+    unsigned long long gated = 0, hits = 0, gletters = 0, mletters = 0;
+
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
+        const idb_tile tile = a.tiles[ti];
+        const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
+        const bool diag = (tile.shape >> 16) & 1u;
+        const bool tvalid = lane < tn;
+        const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
+        idb_entry te{}, qe{};
+        if (tvalid) te = load_entry(a.ent + p);
+        if (lane < qn) qe = load_entry(a.ent + tile.q0 + lane);
+        qrec[lane] = qe;
+        qacc[lane] = 0;
+        tacc[lane] = 0;
+        // rows of the 32-row halves this lane feeds into the MFMAs
+        const uint32_t h = lane & 31u;
+        const uint32_t rq0 = (uint32_t)__shfl((int)qe.row, (int)h, 64), rq1 = (uint32_t)__shfl((int)qe.row, (int)(32 + h), 64);
+        const uint32_t rt0 = (uint32_t)__shfl((int)te.row, (int)h, 64), rt1 = (uint32_t)__shfl((int)te.row, (int)(32 + h), 64);
+        const bool vq0 = h < qn, vq1 = 32 + h < qn, vt0 = h < tn, vt1 = 32 + h < tn;
+        idb_f32x16 sim[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) sim[i][j][r] = 0.f;
+#pragma unroll
+        for (uint32_t kk = 0; kk < 4; ++kk) {
+            const uint32_t k = kk * 16 + (lane >> 5) * 8;
+            const idb_bf16x8 qa0 = load_z8(a.zb, rq0, k, vq0), qa1 = load_z8(a.zb, rq1, k, vq1);
+            const idb_bf16x8 tb0 = load_z8(a.zb, rt0, k, vt0), tb1 = load_z8(a.zb, rt1, k, vt1);
+            sim[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa0, tb0, sim[0][0], 0, 0, 0);
+            sim[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa0, tb1, sim[0][1], 0, 0, 0);
+            sim[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa1, tb0, sim[1][0], 0, 0, 0);
+            sim[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa1, tb1, sim[1][1], 0, 0, 0);
+        }
+        // candidate mask: bit (2 i + j) * 16 + r for output tile (i, j), row r
+        unsigned long long cand = 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint32_t qi = i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    const uint32_t tl = j * 32 + h;
+                    if (sim[i][j][r] >= a.min_sim && qi < qn && tl < tn && (!diag || qi < tl))
+                        cand |= 1ull << ((2 * i + j) * 16 + r);
+                }
+        // target fields of the two target columns this lane holds
+        uint32_t tm[2], tt2[2], tsig[2], ty[2], tp1[2], tp2[2];
+        bool tchk[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int src = (int)(j * 32 + h);
+            const uint32_t lc = (uint32_t)__shfl((int)te.len_chk, src, 64);
+            tm[j] = lc & 0xFFFFu;
+            tchk[j] = (lc >> 16) & 1u;
+            tt2[j] = (uint32_t)__shfl((int)te.t2, src, 64);
+            tsig[j] = (uint32_t)__shfl((int)te.sig, src, 64);
+            ty[j] = (uint32_t)__shfl((int)te.row, src, 64);
+            tp1[j] = (uint32_t)__shfl((int)te.pres1, src, 64);
+            tp2[j] = (uint32_t)__shfl((int)te.pres2, src, 64);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        bool peq_ready = false;
+        uint32_t qlen = 0;
+        auto build_peq = [&]() {
+            for (uint32_t c = 0; c < K; ++c) peq[c * 64 + lane] = 0u;
+            if (tvalid) {
+                const uint4 *codes = (const uint4 *)(a.e_codes + (size_t)p * a.S);
+                const uint32_t m = te.len_chk & 0xFFFFu;
+                for (uint32_t i0 = 0; i0 < m; i0 += 16) {
+                    const uint4 v = codes[i0 >> 4];
+                    const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k) {
+                        const uint32_t i = i0 + k;
+                        if (i < m) peq[((ws[k >> 2] >> (8 * (k & 3))) & 0xFFu) * 64 + lane] |= 1u << (i & 31);
+                    }
+                }
+            }
+            peq_ready = true;
+        };
+        // every lane takes its candidates one per round: gate, the exact
+        // absent-letter bound where the alphabet allows it, then the queue
+        while (__ballot(cand != 0ull)) {
+            bool ok = false;
+            uint32_t qi = 0, tl = 0, lx = 0;
+            if (cand) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(cand);
+                cand &= cand - 1ull;
+                const uint32_t i = b >> 5, j = (b >> 4) & 1u, r = b & 15u;
+                qi = i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                tl = j * 32 + h;
+                const uint32_t m = j ? tm[1] : tm[0];
+                const idb_entry &q = qrec[qi];
+                lx = q.len_chk & 0xFFFFu;
+                const int32_t t = lx >= m ? t_of(q.t2, false) : t_of(j ? tt2[1] : tt2[0], false);
+                const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
+                ok = dl <= t && gate_pair(a, (q.len_chk >> 16) & 1u, j ? tchk[1] : tchk[0], q.sig,
+                                          j ? tsig[1] : tsig[0], q.row, j ? ty[1] : ty[0], tile.bucket);
+                if (ok && a.band_filter) {
+                    const uint32_t p1 = j ? tp1[1] : tp1[0], p2 = j ? tp2[1] : tp2[0];
+                    const uint32_t lbx = __builtin_popcount(q.pres1 & ~p1) + __builtin_popcount(q.pres2 & ~p1);
+                    const uint32_t lby = __builtin_popcount(p1 & ~q.pres1) + __builtin_popcount(p2 & ~q.pres1);
+                    ok = (int32_t)(lbx > lby ? lbx : lby) <= t;
+                }
+            }
+            const unsigned long long pm = __ballot(ok);
+            if (!pm) continue;
+            gated += __popcll(pm);
+            gletters += ok ? lx : 0u;
+            const uint32_t below =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+            if (ok) queue[qlen + below] = (uint16_t)((qi << 8) | tl);
+            qlen += (uint32_t)__popcll(pm);
+            if (qlen >= IDB_TILE) {
+                if (!peq_ready) build_peq();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                myers_batch<W, false>(a, tile, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits,
+                                      mletters);
+                const uint32_t rest = qlen - IDB_TILE;
+                const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                if (lane < rest) queue[lane] = moved;
+                qlen = rest;
+            }
+        }
+        if (qlen) {
+            if (!peq_ready) build_peq();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            myers_batch<W, false>(a, tile, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const int32_t qa = qacc[lane], ta = tacc[lane];
+        if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
+        if (tvalid && ta) atomicAdd(a.cnt + te.row, ta);
+    }
+    finish_edges(a, est, lane);
+    gletters = wave_sum_u64(gletters);
+    mletters = wave_sum_u64(mletters);
+    if (lane == 0 && a.work) {
+        atomicAdd(a.work + 0, gated);
+        atomicAdd(a.work + 1, hits);
+        atomicAdd(a.work + 3, gletters);
+        atomicAdd(a.work + 5, gated);
+        atomicAdd(a.work + 6, mletters);
+    }
+}
+
+// Exact distance of arbitrary row pairs over raw bytes (labelling): one pair
+// per lane, pattern y in W words, match masks built per text letter from y's
+// bytes.  Hamming: mismatches at equal length, UINT32_MAX otherwise.
+template <int W>
+__global__ void k_pair_dist(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const uint2 *pairs,
+                            unsigned long long n, uint32_t ham, uint32_t *dist)
+{
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint2 pr = pairs[i];
+        const uint8_t *x = seq + off[pr.x], *y = seq + off[pr.y];
+        const uint32_t lx = len[pr.x], m = len[pr.y];
+        if (ham) {
+            uint32_t d = 0xFFFFFFFFu;
+            if (lx == m) {
+                d = 0;
+                for (uint32_t k = 0; k < m; ++k) d += x[k] != y[k];
+            }
+            dist[i] = d;
+            continue;
+        }
+        if (m == 0) {
+            dist[i] = lx;
+            continue;
+        }
+        idb_myers<W> st;
+        idb_myers_init<W>(st, m);
+        const uint32_t top_w = (m - 1) >> 5, top_b = (m - 1) & 31;
+        for (uint32_t j = 0; j < lx; ++j) {
+            const uint8_t c = x[j];
+            uint32_t eq[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) eq[w] = 0u;
+            for (uint32_t k = 0; k < m; ++k)
+                if (y[k] == c) eq[k >> 5] |= 1u << (k & 31);
+            idb_myers_step<W>(st, eq, top_w, top_b);
+        }
+        dist[i] = (uint32_t)st.score;
+    }
+}
+
 // Hamming mode: equal lengths only; packed words (2-bit codes for <= 4
 // letters, else 8-bit) compared by XOR, one bit per mismatching field and a
 // bit count.  Query words are preloaded per lane when they fit registers.
@@ -743,7 +971,54 @@ hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
     return a.bits == 2 ? launch_hamming_bits<2, FUSE>(a, s) : launch_hamming_bits<8, FUSE>(a, s);
 }
 
+hipError_t launch_learned(const idb_pair_args &a, hipStream_t s)
+{
+    if (a.n_tiles == 0) return hipSuccess;
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, 1) + IDB_TILE * sizeof(idb_entry);
+    uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
+    if (wpb > 4) wpb = 4;
+    if (wpb == 0) return hipErrorInvalidValue;
+    const size_t shm = per_wave * wpb;
+    const void *k = (const void *)k_pairs_learned;
+    if (shm > 65536) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm)));
+    hipLaunchKernelGGL(k_pairs_learned, grid, block, shm, s, a, wpb);
+    return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_pair_dist_w(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const uint2 *pairs,
+                              unsigned long long n, uint32_t ham, uint32_t *dist, hipStream_t s)
+{
+    unsigned long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_pair_dist<W>, dim3((uint32_t)blocks), dim3(256), 0, s, seq, off, len, pairs, n, ham, dist);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t idb_launch_count_learned(const idb_pair_args &a, hipStream_t s) { return launch_learned(a, s); }
+
+hipError_t idb_launch_pair_distances(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const uint2 *pairs,
+                                     unsigned long long n, uint32_t W, uint32_t ham, uint32_t *dist, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    switch (ham ? 1u : W) {
+    case 1: return launch_pair_dist_w<1>(seq, off, len, pairs, n, ham, dist, s);
+    case 2: return launch_pair_dist_w<2>(seq, off, len, pairs, n, ham, dist, s);
+    case 3: return launch_pair_dist_w<3>(seq, off, len, pairs, n, ham, dist, s);
+    case 4: return launch_pair_dist_w<4>(seq, off, len, pairs, n, ham, dist, s);
+    case 5: return launch_pair_dist_w<5>(seq, off, len, pairs, n, ham, dist, s);
+    case 6: return launch_pair_dist_w<6>(seq, off, len, pairs, n, ham, dist, s);
+    case 7: return launch_pair_dist_w<7>(seq, off, len, pairs, n, ham, dist, s);
+    case 8: return launch_pair_dist_w<8>(seq, off, len, pairs, n, ham, dist, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 size_t idb_lds_limit()
 {

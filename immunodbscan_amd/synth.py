@@ -11,6 +11,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 SYNTH_PATH = os.path.join(PKG, "libidb_synth.so")
 
 NT, AA = 0, 1
+LETTERS = {NT: b"ACGT", AA: b"ACDEFGHIKLMNPQRSTVWY"}  # the generator's alphabets
 LEN_UNIFORM, LEN_NORMAL = 0, 1
 SIZE_GEOMETRIC, SIZE_LOGNORMAL = 0, 1
 MUT_SUB, MUT_EDITS, MUT_SUB_INDEL = 0, 1, 2

@@ -133,6 +133,78 @@ int idb_dbscan(idb_ctx *ctx, const idb_dataset *data, const idb_params *params, 
 int idb_dbscan_batch(idb_ctx *ctx, const idb_dataset *data, const uint32_t *rep_start, uint32_t n_reps,
                      const idb_params *params, idb_outputs *out, uint32_t *n_uniques, idb_stats *stats);
 
+/* ------------------------------------------------------------------------
+ * Exact pair evaluation.  No reference counterpart as entry points; they
+ * expose the reference's own neighbour predicate (R6/R7) for labelling and
+ * measurement.
+ *
+ * idb_pair_distances: the unit-cost Levenshtein (or, with use_hamming, the
+ * Hamming distance, UINT32_MAX for unequal lengths) of each row pair
+ * pairs[2i], pairs[2i+1], computed by the device's Myers kernel.
+ * idb_neighbour_pairs: the unordered unique-row pairs (x < y) with N(x, y)
+ * found by the exact count pass, at most cap of them (*n_pairs gets the total;
+ * IDB_ERR_OOM when it exceeds cap). */
+int idb_pair_distances(idb_ctx *ctx, const idb_dataset *data, const uint32_t *pairs, uint64_t n_pairs,
+                       uint32_t use_hamming, uint32_t *dist);
+int idb_neighbour_pairs(idb_ctx *ctx, const idb_dataset *data, const idb_params *params, uint32_t *pairs,
+                        uint64_t cap, uint64_t *n_pairs);
+
+/* ------------------------------------------------------------------------
+ * Learned candidate filter.  No reference counterpart.  A small 1-D CNN
+ * (token embedding 32 -> 3 x conv1d(k=5, 128 channels, GELU) -> mean over the
+ * row's letters -> linear 128 -> 64 -> L2 norm; bfloat16 weights, float32
+ * accumulation) maps each row (first 32 letters) to a unit vector.  The
+ * learned clustering run proposes, inside the same V/J bucket tiles as the
+ * exact run, the pairs whose embedding dot product reaches min_similarity,
+ * and verifies each proposal with the exact Myers kernel: distances are never
+ * approximated, only pairs the encoder fails to propose can be missed. */
+#define IDB_ENC_PARAMS 193984u    /* float parameters of the encoder            */
+
+typedef struct idb_encoder idb_encoder;
+
+typedef struct {
+    uint32_t epochs;
+    uint32_t batch_pairs;         /* pairs per optimiser step                     */
+    float lr;                     /* Adam learning rate (beta 0.9 / 0.999)        */
+    float margin;                 /* contrastive margin on |z_a - z_b|            */
+    uint64_t seed;                /* pair order shuffling                         */
+    uint32_t log_every;           /* read the loss back every this many steps (0: first/last only) */
+    uint32_t reserved;
+} idb_train_params;
+
+typedef struct {
+    uint32_t steps;
+    float first_loss, last_loss;
+    float mean_sampled_loss_last_epoch;
+    float train_ms;
+} idb_train_stats;
+
+/* vocab maps every byte value to a token id below 32.  Parameters are
+ * initialised from seed (N(0,1) embedding, U(+-1/sqrt(fan_in)) weights). */
+int idb_encoder_create(int device, const uint8_t *vocab, uint64_t seed, idb_encoder **enc);
+void idb_encoder_destroy(idb_encoder *enc);
+const char *idb_encoder_last_error(const idb_encoder *enc);
+uint32_t idb_encoder_param_count(void);
+/* Flat float32 parameters in the order emb[32][32], w1[128][5*32], b1[128],
+ * w2[128][5*128], b2, w3[128][5*128], b3, wp[64][128], bp[64]; convolution
+ * weights are out-channel major with k = tap * c_in + c_in_index. */
+int idb_encoder_get_params(idb_encoder *enc, float *params);
+int idb_encoder_set_params(idb_encoder *enc, const float *params);
+/* z (n_rows x 64 floats) of every row of data. */
+int idb_encoder_embed(idb_encoder *enc, const idb_dataset *data, float *z);
+/* Contrastive loss over the pairs (label 1: neighbours, 0: not) and its
+ * gradient with respect to the flat parameters; no update. */
+int idb_encoder_loss_grad(idb_encoder *enc, const idb_dataset *data, const uint32_t *pairs, const float *labels,
+                          uint32_t n_pairs, float margin, float *loss, float *grad);
+/* Adam training over shuffled batches of the labelled pairs, on the device. */
+int idb_encoder_train(idb_encoder *enc, const idb_dataset *data, const uint32_t *pairs, const float *labels,
+                      uint64_t n_pairs, const idb_train_params *train, idb_train_stats *stats);
+
+/* idb_dbscan with the learned candidate filter (Levenshtein mode, rows of at
+ * most 32 letters).  The encoder must live on the context's device. */
+int idb_dbscan_learned(idb_ctx *ctx, idb_encoder *enc, float min_similarity, const idb_dataset *data,
+                       const idb_params *params, idb_outputs *out, idb_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def declared_functions() -> list[str]:
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(idb_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char \*)\s*(idb_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
