@@ -22,6 +22,8 @@ IDB_ERR_UNSUPPORTED = -4
 IDB_MEM_HOST = 0
 IDB_MEM_DEVICE = 1
 IDB_N_STAGES = 8
+IDB_FLAG_AUTOTUNE = 1
+IDB_FLAG_GRAPH = 2
 
 STAGE_NAMES = [
     "Identify Duplicates",
@@ -82,7 +84,7 @@ class IdbParams(C.Structure):
         ("tree_stats", C.c_uint32),
         ("debug_hash_bits", C.c_uint32),
         ("edge_capacity", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -121,6 +123,12 @@ class IdbStats(C.Structure):
         ("fuse_kernel_ms", C.c_float),
         ("tree_depth", C.c_uint32 * 51),
         ("stage_ms", C.c_float * IDB_N_STAGES),
+        ("tuned_wpb", C.c_uint32),
+        ("tuned_grid_mul", C.c_uint32),
+        ("tuned_xcd_order", C.c_uint32),
+        ("tuned_trials", C.c_uint32),
+        ("graph_used", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -256,7 +264,8 @@ class Context:
 
 
 def make_params(eps: float = 0.15, abs_k: int = -1, min_pts: int = 2, use_hamming: bool = False,
-                tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0) -> IdbParams:
+                tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0,
+                flags: int = 0) -> IdbParams:
     p = IdbParams()
     p.eps = float(np.float32(eps))
     p.abs_k = int(abs_k)
@@ -265,6 +274,7 @@ def make_params(eps: float = 0.15, abs_k: int = -1, min_pts: int = 2, use_hammin
     p.tree_stats = 1 if tree_stats else 0
     p.debug_hash_bits = int(debug_hash_bits)
     p.edge_capacity = int(edge_capacity)
+    p.flags = int(flags)
     return p
 
 

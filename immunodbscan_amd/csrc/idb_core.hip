@@ -26,6 +26,8 @@
 #include <cstdio>
 #include <string>
 #include <string_view>
+#include <array>
+#include <map>
 #include <vector>
 
 #include "../../include/idb.h"
@@ -40,7 +42,7 @@ enum Slot {
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
-    S_DIST,
+    S_DIST, S_CNT2,
     S_COUNT
 };
 
@@ -53,6 +55,12 @@ struct idb_ctx {
     uint64_t *pinned = nullptr;   // small host staging area for counters
     hipEvent_t ev[IDB_N_STAGES + 1] = {};
     hipEvent_t kev[4] = {};       // around the count and the fuse kernels
+    // autotuner: best count-kernel launch shape per kernel and shape class
+    struct Tuned {
+        uint32_t wpb, grid_mul, no_xcd, trials;
+    };
+    std::map<std::array<uint32_t, 6>, Tuned> tuned;
+    hipGraphExec_t gexec = nullptr;  // the tail graph, updated in place
 };
 
 namespace {
@@ -633,6 +641,11 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t x)
     return x;
 }
 
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // R9/R10: label root of every unique row (its component root, its border
 // core's root, or none) and root flags for R12.
 __global__ void k_roots(const uint32_t *uniq, const uint32_t *num, const int32_t *core, const uint32_t *par,
@@ -747,6 +760,18 @@ void exclusive_scan_total(idb_ctx *c, const T *in, T *out, size_t n_plus_1)
     check(c, rocprim::exclusive_scan(t, bytes, in, out, T(0), n_plus_1, rocprim::plus<T>(), c->stream), "scan");
 }
 
+// Size the arena's temporary slot for the renumbering scan ahead of a graph
+// capture (no allocation may happen while capturing).
+void reserve_scan_tmp(idb_ctx *c, size_t n_plus_1)
+{
+    size_t bytes = 0;
+    const uint32_t *in = nullptr;
+    uint32_t *out = nullptr;
+    check(c, rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, n_plus_1, rocprim::plus<uint32_t>(), c->stream),
+          "scan size");
+    (void)tmp(c, bytes);
+}
+
 void max_scan(idb_ctx *c, const uint32_t *in, uint32_t *out, size_t n)
 {
     size_t bytes = 0;
@@ -787,6 +812,60 @@ unsigned bits_for(uint64_t v)
 }
 
 void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "event record"); }
+
+// Autotuner of the count kernel's launch shape: waves per block (1, 2, 4 as
+// LDS allows), grid of 1x or 2x the resident blocks, XCD-grouped or plain
+// block order.  Each candidate runs the real count pass on the real tiles
+// (counts restored from a snapshot in between) and the fastest by HIP events
+// is kept for this kernel and shape class (kernel, words, code width, filter,
+// alphabet, log2 tiles) for the context's lifetime.
+void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bool learned, uint32_t *small,
+              unsigned long long *work)
+{
+    uint32_t lt = 0;
+    while (lt < 31 && (1u << (lt + 1)) <= a.n_tiles) ++lt;
+    const std::array<uint32_t, 6> key{learned ? 2u : (ham ? 1u : 0u), ham ? a.HW : W, a.bits, a.band_filter, a.K, lt};
+    auto it = c->tuned.find(key);
+    if (it == c->tuned.end()) {
+        hipStream_t s = c->stream;
+        int32_t *snap = buf<int32_t>(c, S_CNT2, N);
+        check(c, hipMemcpyAsync(snap, a.cnt, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune snapshot");
+        idb_ctx::Tuned best{0, 0, 0, 0};
+        float best_ms = 1e30f;
+        uint32_t trials = 0;
+        for (uint32_t wpb : {1u, 2u, 4u})
+            for (uint32_t mul : {1u, 2u})
+                for (uint32_t nx : {0u, 1u}) {
+                    a.tune_wpb = wpb;
+                    a.tune_grid_mul = mul;
+                    a.no_xcd_order = nx;
+                    check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
+                    check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
+                    check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+                    check(c, hipEventRecord(c->kev[0], s), "event record");
+                    const hipError_t e = learned ? idb_launch_count_learned(a, s)
+                                                 : (ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s));
+                    check(c, e, "autotune launch");
+                    check(c, hipEventRecord(c->kev[1], s), "event record");
+                    check(c, hipEventSynchronize(c->kev[1]), "event sync");
+                    float ms = 0.f;
+                    check(c, hipEventElapsedTime(&ms, c->kev[0], c->kev[1]), "event time");
+                    ++trials;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = {wpb, mul, nx, 0};
+                    }
+                }
+        best.trials = trials;
+        check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
+        check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
+        check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+        it = c->tuned.emplace(key, best).first;
+    }
+    a.tune_wpb = it->second.wpb;
+    a.tune_grid_mul = it->second.grid_mul;
+    a.no_xcd_order = it->second.no_xcd;
+}
 
 // Options beyond the reference's run: the learned candidate filter, and a
 // copy of the neighbour pairs for the caller.
@@ -1048,90 +1127,139 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             a.zb = zb;
             a.min_sim = x->min_sim;
         }
-        check(c, hipEventRecord(c->kev[0], s), "event record");
-        if (learned) check(c, idb_launch_count_learned(a, s), "learned count kernel launch");
-        else check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
-        check(c, hipEventRecord(c->kev[1], s), "event record");
     }
     int32_t *core = buf<int32_t>(c, S_CORE, N);
-    hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, small);
-    record(c, 3);
-
-    // --- Grow clusters: union over the recorded neighbour pairs, or a fuse
-    // pass over the tiles when the pair list overflowed
-    bool from_edges = false;
-    if (E > 0) {
-        uint32_t n_chunks = 0;
-        readback(c, &n_chunks, small + 12, sizeof(n_chunks));
-        from_edges = n_chunks <= a.max_chunks;
-        if (learned && !from_edges)
-            fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
-        if (x && x->pairs_out) {
-            if (!from_edges) fail(c, IDB_ERR_OOM, "neighbour pair list overflow: raise edge_capacity");
-            std::vector<uint32_t> fill(n_chunks);
-            std::vector<uint2> ed((size_t)n_chunks << a.chunk_log2);
-            if (n_chunks) {
-                check(c, hipMemcpyAsync(fill.data(), a.chunk_fill, sizeof(uint32_t) * n_chunks, hipMemcpyDeviceToHost, s),
-                      "pairs copy");
-                check(c, hipMemcpyAsync(ed.data(), a.edges, sizeof(uint2) * ed.size(), hipMemcpyDeviceToHost, s),
-                      "pairs copy");
-                check(c, hipStreamSynchronize(s), "pairs sync");
-            }
-            uint64_t k = 0;
-            for (uint32_t ch = 0; ch < n_chunks; ++ch)
-                for (uint32_t i = 0; i < fill[ch]; ++i, ++k) {
-                    if (k >= x->pairs_cap) continue;
-                    const uint2 e = ed[((size_t)ch << a.chunk_log2) + i];
-                    x->pairs_out[2 * k] = std::min(e.x, e.y);
-                    x->pairs_out[2 * k + 1] = std::max(e.x, e.y);
-                }
-            *x->n_pairs_out = k;
-        }
-        check(c, hipEventRecord(c->kev[2], s), "event record");
-        if (from_edges) {
-            check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
-        } else {
-            a.core = core;
-            a.par = par;
-            a.bmin = bmin;
-            a.cnt = nullptr;
-            a.edges = nullptr;
-            check(c, ham ? idb_launch_fuse_hamming(a, s) : idb_launch_fuse_myers(a, W, s), "fuse kernel launch");
-        }
-        check(c, hipEventRecord(c->kev[3], s), "event record");
-    }
-    record(c, 4);
-
-    // --- Reduction of cluster information tree structure
     uint32_t *lab = buf<uint32_t>(c, S_LAB, N);
-    hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag);
-    uint32_t *hist = nullptr;
-    if (p->tree_stats) {
-        hist = small + 16;
-        hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, hist);
-    }
-    record(c, 5);
-
-    // --- Re-number clusters and identify noise
-    check(c, hipMemsetAsync(rflag + N, 0, sizeof(uint32_t), s), "memset");
-    exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);
     const bool dev_out = o->mem == IDB_MEM_DEVICE;
     uint32_t *cid = dev_out ? o->clone_id : buf<uint32_t>(c, S_OUT_CID, N);
     int32_t *core_o = dev_out ? o->core : buf<int32_t>(c, S_OUT_CORE, N);
     uint32_t *cop_o = dev_out ? o->copies : buf<uint32_t>(c, S_OUT_COP, N);
     uint32_t *unq_o = dev_out ? o->uniques : buf<uint32_t>(c, S_OUT_UNQ, N);
     uint32_t *cnt_o = o->neighbour_count ? (dev_out ? o->neighbour_count : buf<uint32_t>(c, S_OUT_CNT, N)) : nullptr;
-    hipLaunchKernelGGL(k_labels, dim3(blocks_for(N)), dim3(256), 0, s, rep, lab, rank, rid, rstart, core, copies, cnt, N,
-                       cid, core_o, cop_o, cnt_o, small);
-    check(c, hipMemsetAsync(unq_o, 0, sizeof(uint32_t) * N, s), "memset");
-    uint32_t *ustart = nullptr;
-    if (n_reps > 0) {
-        ustart = buf<uint32_t>(c, S_USTART, n_reps + 1);
-        hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart, n_reps,
-                           ustart);
+    uint32_t *ustart = n_reps > 0 ? buf<uint32_t>(c, S_USTART, n_reps + 1) : nullptr;
+    reserve_scan_tmp(c, (size_t)N + 1);
+    if (E > 0 && T > 0 && (p->flags & IDB_FLAG_AUTOTUNE)) autotune(c, a, N, W, ham, learned, small, work);
+    const bool graph = (p->flags & IDB_FLAG_GRAPH) && !(x && x->pairs_out);
+    bool from_edges = false;
+
+    // Everything after the sizing steps.  As a graph (g) the union / fuse
+    // choice is made on the device (each kernel checks the chunk counter) and
+    // no event is recorded inside.
+    auto tail = [&](bool g) {
+        if (E > 0) {
+            if (!g) check(c, hipEventRecord(c->kev[0], s), "event record");
+            if (learned) check(c, idb_launch_count_learned(a, s), "learned count kernel launch");
+            else check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
+            if (!g) check(c, hipEventRecord(c->kev[1], s), "event record");
+        }
+        hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, small);
+        if (!g) record(c, 3);
+
+        // --- Grow clusters: union over the recorded neighbour pairs, or a
+        // fuse pass over the tiles when the pair list overflowed
+        if (E > 0) {
+            idb_pair_args f = a;
+            f.core = core;
+            f.par = par;
+            f.bmin = bmin;
+            f.cnt = nullptr;
+            f.edges = nullptr;
+            if (g) {
+                check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
+                if (!learned) {
+                    f.fuse_if_overflow = 1;
+                    check(c, ham ? idb_launch_fuse_hamming(f, s) : idb_launch_fuse_myers(f, W, s), "fuse kernel launch");
+                }
+            } else {
+                uint32_t n_chunks = 0;
+                readback(c, &n_chunks, small + 12, sizeof(n_chunks));
+                from_edges = n_chunks <= a.max_chunks;
+                if (learned && !from_edges)
+                    fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
+                if (x && x->pairs_out) {
+                    if (!from_edges) fail(c, IDB_ERR_OOM, "neighbour pair list overflow: raise edge_capacity");
+                    std::vector<uint32_t> fill(n_chunks);
+                    std::vector<uint2> ed((size_t)n_chunks << a.chunk_log2);
+                    if (n_chunks) {
+                        check(c, hipMemcpyAsync(fill.data(), a.chunk_fill, sizeof(uint32_t) * n_chunks,
+                                                hipMemcpyDeviceToHost, s), "pairs copy");
+                        check(c, hipMemcpyAsync(ed.data(), a.edges, sizeof(uint2) * ed.size(), hipMemcpyDeviceToHost, s),
+                              "pairs copy");
+                        check(c, hipStreamSynchronize(s), "pairs sync");
+                    }
+                    uint64_t k = 0;
+                    for (uint32_t ch = 0; ch < n_chunks; ++ch)
+                        for (uint32_t i = 0; i < fill[ch]; ++i, ++k) {
+                            if (k >= x->pairs_cap) continue;
+                            const uint2 e = ed[((size_t)ch << a.chunk_log2) + i];
+                            x->pairs_out[2 * k] = std::min(e.x, e.y);
+                            x->pairs_out[2 * k + 1] = std::max(e.x, e.y);
+                        }
+                    *x->n_pairs_out = k;
+                }
+                check(c, hipEventRecord(c->kev[2], s), "event record");
+                if (from_edges) check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
+                else check(c, ham ? idb_launch_fuse_hamming(f, s) : idb_launch_fuse_myers(f, W, s), "fuse kernel launch");
+                check(c, hipEventRecord(c->kev[3], s), "event record");
+            }
+        }
+        if (!g) record(c, 4);
+
+        // --- Reduction of cluster information tree structure
+        hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag);
+        if (p->tree_stats)
+            hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, small + 16);
+        if (!g) record(c, 5);
+
+        // --- Re-number clusters and identify noise
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rflag + N, 1u, 0u);
+        exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);
+        hipLaunchKernelGGL(k_labels, dim3(blocks_for(N)), dim3(256), 0, s, rep, lab, rank, rid, rstart, core, copies, cnt,
+                           N, cid, core_o, cop_o, cnt_o, small);
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N)), dim3(256), 0, s, unq_o, N, 0u);
+        if (n_reps > 0)
+            hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart,
+                               n_reps, ustart);
+        hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart, unq_o);
+        check(c, hipGetLastError(), "tail launch");
+        if (!g) record(c, 6);
+    };
+
+    if (graph) {
+        hipGraph_t gr = nullptr;
+        check(c, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+        try {
+            tail(true);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        check(c, hipStreamEndCapture(s, &gr), "end capture");
+        bool updated = false;
+        if (c->gexec) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(c->gexec, gr, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            (void)hipGetLastError();
+            if (!updated) {
+                (void)hipGraphExecDestroy(c->gexec);
+                c->gexec = nullptr;
+            }
+        }
+        if (!c->gexec) {
+            const hipError_t e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                check(c, e, "graph instantiate");
+            }
+        }
+        (void)hipGraphDestroy(gr);
+        check(c, hipGraphLaunch(c->gexec, s), "graph launch");
+        for (int k = 3; k <= 6; ++k) record(c, k);
+    } else {
+        tail(false);
     }
-    hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart, unq_o);
-    record(c, 6);
     if (!dev_out) {
         check(c, hipMemcpyAsync(o->clone_id, cid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
         check(c, hipMemcpyAsync(o->core, core_o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
@@ -1152,6 +1280,11 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
+    if (graph && E > 0) {
+        from_edges = fin[12] <= a.max_chunks;
+        if (learned && !from_edges)
+            fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
+    }
     if (n_uniques_h)
         for (uint32_t r = 0; r < n_reps; ++r) n_uniques_h[r] = us[r + 1] - us[r];
     if (st) {
@@ -1171,7 +1304,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->myers_letters = wk[6];
         st->band_filter = a.band_filter;
         st->fuse_from_edges = from_edges ? 1u : 0u;
-        if (E > 0 && T > 0) {
+        st->tuned_wpb = a.tune_wpb;
+        st->tuned_grid_mul = a.tune_grid_mul;
+        st->tuned_xcd_order = a.tune_wpb ? (a.no_xcd_order ? 0u : 1u) : 0u;
+        st->tuned_trials = 0;
+        if (p->flags & IDB_FLAG_AUTOTUNE)
+            for (const auto &kv : c->tuned)
+                if (kv.second.wpb == a.tune_wpb && kv.second.grid_mul == a.tune_grid_mul) st->tuned_trials = kv.second.trials;
+        st->graph_used = graph ? 1u : 0u;
+        if (E > 0 && T > 0 && !graph) {
             (void)hipEventElapsedTime(&st->count_kernel_ms, c->kev[0], c->kev[1]);
             (void)hipEventElapsedTime(&st->fuse_kernel_ms, c->kev[2], c->kev[3]);
         }
@@ -1242,6 +1383,7 @@ void idb_destroy(idb_ctx *c)
     for (int k = 0; k < 4; ++k)
         if (c->kev[k]) (void)hipEventDestroy(c->kev[k]);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -81,6 +81,9 @@ struct idb_pair_args {
     uint32_t fuse_if_overflow;    // fuse pass: run only when the pair list overflowed
     const uint16_t *zb;           // learned filter: bfloat16 embedding (64) per row
     float min_sim;                // learned filter: proposal threshold on z_x . z_y
+    // launch shape (0 = default): waves per block, grid = resident blocks x
+    // grid_mul, and plain block order instead of the XCD-grouped one
+    uint32_t tune_wpb, tune_grid_mul, no_xcd_order;
 };
 
 // LDS bytes one wave of the Myers kernels needs: Peq columns, the edge

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     int32_t *tacc = qacc + IDB_TILE;
     if (FUSE && fuse_skipped(a)) return;
     const uint32_t total_waves = gridDim.x * wpb;
-    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
@@ -510,8 +510,8 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
     int32_t *tacc = qacc + IDB_TILE;
     idb_entry *qrec = (idb_entry *)(tacc + IDB_TILE);
     const uint32_t total_waves = gridDim.x * wpb;
-    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
-    unsigned long long gated = 0, hits = 0, gletters = 0, mletters = 0;
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
+    unsigned long long gated = 0, hits = 0, gletters = 0, mletters = 0, mpairs = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
         const idb_tile tile = a.tiles[ti];
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
         // every lane takes its candidates one per round: gate, the exact
         // absent-letter bound where the alphabet allows it, then the queue
         while (__ballot(cand != 0ull)) {
-            bool ok = false;
+            bool ok = false, proposed = false;
             uint32_t qi = 0, tl = 0, lx = 0;
             if (cand) {
                 const uint32_t b = (uint32_t)__builtin_ctzll(cand);
@@ -613,6 +613,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
                 const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
                 ok = dl <= t && gate_pair(a, (q.len_chk >> 16) & 1u, j ? tchk[1] : tchk[0], q.sig,
                                           j ? tsig[1] : tsig[0], q.row, j ? ty[1] : ty[0], tile.bucket);
+                proposed = ok;
                 if (ok && a.band_filter) {
                     const uint32_t p1 = j ? tp1[1] : tp1[0], p2 = j ? tp2[1] : tp2[0];
                     const uint32_t lbx = __builtin_popcount(q.pres1 & ~p1) + __builtin_popcount(q.pres2 & ~p1);
@@ -620,10 +621,11 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
                     ok = (int32_t)(lbx > lby ? lbx : lby) <= t;
                 }
             }
+            gated += __popcll(__ballot(proposed));
+            gletters += proposed ? lx : 0u;
             const unsigned long long pm = __ballot(ok);
             if (!pm) continue;
-            gated += __popcll(pm);
-            gletters += ok ? lx : 0u;
+            mpairs += __popcll(pm);
             const uint32_t below =
                 __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
             if (ok) queue[qlen + below] = (uint16_t)((qi << 8) | tl);
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
         atomicAdd(a.work + 0, gated);
         atomicAdd(a.work + 1, hits);
         atomicAdd(a.work + 3, gletters);
-        atomicAdd(a.work + 5, gated);
+        atomicAdd(a.work + 5, mpairs);
         atomicAdd(a.work + 6, mletters);
     }
 }
@@ -712,7 +714,7 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t total_waves = gridDim.x * wpb;
-    const uint32_t wave0 = xcd_block(blockIdx.x, gridDim.x) * wpb + wib;
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
     const uint32_t HW = a.HW;
     __shared__ uint2 stage[4][IDB_TILE];
     __shared__ int32_t qacc_s[4][IDB_TILE];
@@ -896,11 +898,19 @@ uint32_t resident_blocks(const void *kernel, uint32_t block, size_t shm)
     return (uint32_t)per_cu * (uint32_t)cus;
 }
 
-uint32_t grid_for(uint32_t n_tiles, uint32_t wpb, uint32_t resident)
+uint32_t grid_for(uint32_t n_tiles, uint32_t wpb, uint32_t resident, uint32_t mul = 1)
 {
     uint32_t b = (n_tiles + wpb - 1) / wpb;
-    if (b > resident) b = resident;
+    const uint32_t cap = resident * (mul ? mul : 1u);
+    if (b > cap) b = cap;
     return b ? b : 1u;
+}
+
+// Waves per block: the tuned value when set, within [1, most].
+uint32_t pick_wpb(const idb_pair_args &a, uint32_t most)
+{
+    uint32_t w = a.tune_wpb ? a.tune_wpb : most;
+    return std::max(1u, std::min(w, most));
 }
 
 template <int W, bool FUSE, bool FILTER>
@@ -911,7 +921,7 @@ hipError_t launch_myers_w(const idb_pair_args &a, dim3 block, size_t shm, uint32
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const dim3 grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm)));
+    const dim3 grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm), a.tune_grid_mul));
     hipLaunchKernelGGL((k_pairs_myers<W, FUSE, FILTER>), grid, block, shm, s, a, wpb);
     return hipGetLastError();
 }
@@ -926,6 +936,7 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
     if (wpb > 4) wpb = 4;
     if (wpb == 0) wpb = per_wave <= limit ? 1u : 0u;
     if (wpb == 0) return hipErrorInvalidValue;
+    wpb = pick_wpb(a, wpb);
     const dim3 block(64 * wpb);
     const size_t shm = per_wave * wpb;
     if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, true>(a, block, shm, wpb, s);
@@ -945,9 +956,9 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
 template <int HWMAX, int BITS, bool FUSE>
 hipError_t launch_hamming_hw(const idb_pair_args &a, hipStream_t s)
 {
-    const uint32_t wpb = 4;
+    const uint32_t wpb = pick_wpb(a, 4);
     const void *k = (const void *)k_pairs_hamming<HWMAX, BITS, FUSE>;
-    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, 64 * wpb, 0)));
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, 64 * wpb, 0), a.tune_grid_mul));
     hipLaunchKernelGGL((k_pairs_hamming<HWMAX, BITS, FUSE>), grid, block, 0, s, a, wpb);
     return hipGetLastError();
 }
@@ -978,13 +989,14 @@ hipError_t launch_learned(const idb_pair_args &a, hipStream_t s)
     uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
     if (wpb > 4) wpb = 4;
     if (wpb == 0) return hipErrorInvalidValue;
+    wpb = pick_wpb(a, wpb);
     const size_t shm = per_wave * wpb;
     const void *k = (const void *)k_pairs_learned;
     if (shm > 65536) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm)));
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm), a.tune_grid_mul));
     hipLaunchKernelGGL(k_pairs_learned, grid, block, shm, s, a, wpb);
     return hipGetLastError();
 }

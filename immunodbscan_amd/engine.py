@@ -31,9 +31,10 @@ def default_context() -> _lib.Context:
     return _default_ctx
 
 
-def _params(cl: Clustering, tree_stats: bool, debug_hash_bits: int, edge_capacity: int = 0) -> _lib.IdbParams:
+def _params(cl: Clustering, tree_stats: bool, debug_hash_bits: int, edge_capacity: int = 0,
+            flags: int = 0) -> _lib.IdbParams:
     return _lib.make_params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, tree_stats, debug_hash_bits,
-                            edge_capacity)
+                            edge_capacity, flags)
 
 
 def _dataset(rep: Repertoire) -> _lib.IdbDataset:
@@ -43,7 +44,7 @@ def _dataset(rep: Repertoire) -> _lib.IdbDataset:
 
 
 def dbscan(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True,
-           tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0) -> Result:
+           tree_stats: bool = False, debug_hash_bits: int = 0, edge_capacity: int = 0, flags: int = 0) -> Result:
     ctx = ctx or default_context()
     n = rep.n
     cid = np.zeros(n, np.uint32)
@@ -54,13 +55,13 @@ def dbscan(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, cou
     ds = _dataset(rep)
     outs = _lib.make_outputs(cid, core, cop, unq, cnt)
     st = _lib.IdbStats()
-    prm = _params(cl, tree_stats, debug_hash_bits, edge_capacity)
+    prm = _params(cl, tree_stats, debug_hash_bits, edge_capacity, flags)
     ctx.check(_lib.lib.idb_dbscan(ctx.handle, ds, prm, outs, st))
     return Result(cid, core, cop, unq, int(st.n_uniques), cnt, st.as_dict())
 
 
 def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = True,
-                 edge_capacity: int = 0) -> Result:
+                 edge_capacity: int = 0, flags: int = 0) -> Result:
     ctx = ctx or default_context()
     n = rep.n
     starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
@@ -74,7 +75,7 @@ def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = Non
     ds = _dataset(rep)
     outs = _lib.make_outputs(cid, core, cop, unq, cnt)
     st = _lib.IdbStats()
-    prm = _params(cl, False, 0, edge_capacity)
+    prm = _params(cl, False, 0, edge_capacity, flags)
     ctx.check(_lib.lib.idb_dbscan_batch(ctx.handle, ds, starts.ctypes.data, nr, prm, outs, nu.ctypes.data, st))
     return Result(cid, core, cop, unq, nu[:nr], cnt, st.as_dict())
 
@@ -83,7 +84,8 @@ class DeviceRun:
     """A repertoire resident in HBM with device-side outputs, for timing the
     path without host copies.  Buffers come from immunodbscan_amd.hip."""
 
-    def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, batch: bool = False):
+    def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, batch: bool = False,
+                 flags: int = 0):
         from .hip import DeviceBuffer, synchronize
 
         self.ctx = ctx or default_context()
@@ -101,7 +103,7 @@ class DeviceRun:
         self.ds = _lib.make_dataset(n, self.seq, self.off, self.len, self.vb, rep.vw, self.jb, rep.jw,
                                     mem=_lib.IDB_MEM_DEVICE, seq_bytes=len(rep.seq))
         self.outs = _lib.make_outputs(self.cid, self.core, self.cop, self.unq, self.cnt, mem=_lib.IDB_MEM_DEVICE)
-        self.prm = _params(cl, False, 0)
+        self.prm = _params(cl, False, 0, 0, flags)
         self.batch = batch
         self.starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
         self.nu = np.zeros(max(len(self.starts) - 1, 1), np.uint32)

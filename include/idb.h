@@ -65,8 +65,17 @@ typedef struct {
                                      many bits (forces collisions in tests)                   */
     uint32_t edge_capacity;       /* 0 = automatic; else the most neighbour pairs kept from the
                                      count pass (more fall back to a second pass over pairs)  */
-    uint32_t reserved;
+    uint32_t flags;               /* IDB_FLAG_* below (0: plain launches)                      */
 } idb_params;
+
+/* flags: AUTOTUNE times the count kernel's launch shapes (waves per block,
+ * grid size, XCD tile order) on the first run of each kernel and shape class
+ * in a context and keeps the fastest; GRAPH captures everything after the
+ * sizing steps (count pass, core flags, cluster growth, renumbering, outputs)
+ * into one HIP graph per context, updated in place on later runs, with the
+ * union / fuse choice made on the device.  Neither changes any result. */
+#define IDB_FLAG_AUTOTUNE 1u
+#define IDB_FLAG_GRAPH 2u
 
 /* Per-row outputs (cuda/Interlink.hpp:48-49 and :59-67).  uniques holds the
  * ascending representative indices in its first n_uniques entries and 0 after.
@@ -100,7 +109,10 @@ typedef struct {
     float count_kernel_ms;        /* count-pass kernel, HIP events on the library stream */
     float fuse_kernel_ms;         /* fuse-pass kernel                                */
     uint32_t tree_depth[51];      /* union-find depth histogram when tree_stats      */
-    float stage_ms[IDB_N_STAGES];
+    float stage_ms[IDB_N_STAGES]; /* with IDB_FLAG_GRAPH the graph's time is the core-point stage's */
+    uint32_t tuned_wpb, tuned_grid_mul, tuned_xcd_order, tuned_trials;  /* count-kernel launch shape */
+    uint32_t graph_used;          /* 1: the tail ran as one graph launch             */
+    uint32_t reserved;
 } idb_stats;
 
 /* Context on HIP device `device`: a stream and a reusable device arena.
