@@ -13,6 +13,18 @@ Metric: pairs/sec under the reference's all-pairs work model, N(N-1)/2 per
 step divided by the step time; seqs clustered/sec is reported beside it.
 The CPU baseline is the oracle's literal all-pairs restatement of the
 reference, run on this host on the first rows of the same workload.
+
+Beside the headline exact line the JSON carries:
+  learned   the learned candidate filter on the same workload and on a
+            2,000,000-row repertoire of the same generator: the encoder is
+            trained on the device (untimed) on pairs labelled by the exact
+            kernels of a separate repertoire, its threshold is tuned for
+            99.9% neighbour-pair recall on a third one, and the timed step is
+            the whole learned clustering call (embedding included); recall,
+            ARI and the speedup over the exact path are reported;
+  batched   512 repertoires (about 8.6M nucleotide rows, Hamming, eps 0.15,
+            minPts 2) in one batched call with the autotuned, graph-captured
+            tail; pairs/s is the sum of every repertoire's N(N-1)/2.
 """
 from __future__ import annotations
 
@@ -53,6 +65,123 @@ def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
     return ops
 
 
+ENC_FLOPS_PER_ROW = 2 * 32 * (5 * 32 * 128 + 2 * 5 * 128 * 128)  # three conv layers, 32 positions
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def timed(run, steps: int, warmup: int):
+    """ms per call of run() over `steps` calls after `warmup`, bracketed by
+    device synchronisation of both runtimes."""
+    import torch
+
+    from immunodbscan_amd import hip
+
+    st = None
+    for _ in range(warmup):
+        st = run()
+    hip.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = run()
+    hip.synchronize()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, st
+
+
+def learned_section(args, cl, exact_rep, exact_ms, exact_res) -> dict:
+    from immunodbscan_amd import configs, engine, learned, synth
+
+    t0 = time.perf_counter()
+    vocab = learned.vocab_for(synth.LETTERS[synth.AA])
+    train = synth.generate(configs.c2_params(n_rows=200_000, n_seeds=20_000, seed=41))
+    pairs, labels = learned.training_pairs(train, cl, args.train_pairs, seed=3)
+    enc = learned.Encoder(vocab, seed=2)
+    tp = learned.TrainParams(epochs=args.train_epochs, batch_pairs=4096, lr=5e-4, margin=0.9)
+    tstats = enc.train(train, pairs, labels, tp)
+    tune = synth.generate(configs.c2_params(n_rows=200_000, n_seeds=20_000, seed=43))
+    tau = learned.tune_threshold(enc.embed(tune), learned.neighbour_pairs(tune, cl), recall=0.999)
+    log(f"learned: trained {tstats['steps']} steps in {tstats['train_ms']:.0f} ms, tau {tau:.4f} "
+        f"({time.perf_counter() - t0:.1f} s with labelling)")
+    out = {
+        "training": {"pairs": int(len(labels)), "positive_fraction": float(labels.mean()), **tstats,
+                     "labelled_by": "exact count pass (neighbours) + exact Myers distances (sampled non-neighbours)"},
+        "min_similarity": tau,
+        "threshold_rule": "0.1% quantile of neighbour-pair similarities on a separate 200k-row repertoire",
+    }
+
+    def one(rep, ex_ms, ex_res, tag):
+        lrun = engine.DeviceRun(rep, cl, encoder=enc, min_sim=tau)
+        ms, st = timed(lrun, args.steps, args.warmup)
+        res = lrun.result()
+        s = res.stats
+        n = rep.n
+        enc_ms = s["encoder_ms"]
+        flops = ENC_FLOPS_PER_ROW * n
+        return {
+            "workload": tag,
+            "rows": n,
+            "ms_per_step": ms,
+            "pairs_per_sec": n * (n - 1) / 2 / (ms * 1e-3),
+            "seqs_clustered_per_sec": n / (ms * 1e-3),
+            "exact_ms_per_step": ex_ms,
+            "speedup_vs_exact": ex_ms / ms,
+            "pair_recall": s["neighbour_pairs"] / max(ex_res.stats["neighbour_pairs"], 1),
+            "ari": learned.ari(ex_res.clone_id, res.clone_id),
+            "proposals": s["gated_pairs"],
+            "exact_gated_pairs": ex_res.stats["gated_pairs"],
+            "myers_pairs": s["myers_pairs"],
+            "exact_myers_pairs": ex_res.stats["myers_pairs"],
+            "count_kernel_ms": s["count_kernel_ms"],
+            "exact_count_kernel_ms": ex_res.stats["count_kernel_ms"],
+            "encoder_ms": enc_ms,
+            "encoder_roofline": {"bound": "mfma", "achieved": flops / (enc_ms * 1e-3) / 1e12,
+                                 "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": flops / (enc_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+                                 "flops_per_row": ENC_FLOPS_PER_ROW},
+        }
+
+    out["c2"] = one(exact_rep, exact_ms, exact_res, "same as the headline line")
+    log(f"learned c2: {out['c2']['ms_per_step']:.3f} ms/step, recall {out['c2']['pair_recall']:.5f}")
+    if args.c4_rows:
+        big = synth.generate(configs.c4_params(n_rows=args.c4_rows))
+        erun = engine.DeviceRun(big, cl)
+        ems, _ = timed(erun, max(2, args.steps // 4), 1)
+        eres = erun.result()
+        out["c4"] = one(big, ems, eres, f"{big.n} aa CDR3s, {args.c4_rows // 10} seeds, t=2 edits, minPts=3")
+        log(f"learned c4: {out['c4']['ms_per_step']:.3f} ms/step vs exact {ems:.3f}")
+    return out
+
+
+def batched_section(args) -> dict:
+    from immunodbscan_amd import _lib, configs, engine, synth
+
+    reps = configs.c3_repertoires(n_reps=args.c3_reps)
+    cat = synth.concat(reps)
+    flags = _lib.IDB_FLAG_AUTOTUNE | _lib.IDB_FLAG_GRAPH
+    run = engine.DeviceRun(cat, configs.C3, batch=True, flags=flags)
+    ms, st = timed(run, max(3, args.steps // 2), 2)
+    s = st.as_dict()
+    pairs = float(sum(r.n * (r.n - 1) / 2 for r in reps))
+    log(f"batched: {len(reps)} repertoires, {cat.n} rows, {ms:.3f} ms/step")
+    return {
+        "workload": f"{len(reps)} nt repertoires, lognormal sizes (median 5k, 200-200k), Hamming eps 0.15, minPts 2",
+        "repertoires": len(reps),
+        "rows": cat.n,
+        "ms_per_step": ms,
+        "pairs_per_sec": pairs / (ms * 1e-3),
+        "seqs_clustered_per_sec": cat.n / (ms * 1e-3),
+        "gated_pairs": s["gated_pairs"],
+        "neighbour_pairs": s["neighbour_pairs"],
+        "buckets": s["n_buckets"],
+        "tiles": s["tiles"],
+        "graph_used": s["graph_used"],
+        "tuned_launch": {"waves_per_block": s["tuned_wpb"], "grid_x_resident": s["tuned_grid_mul"],
+                         "xcd_order": s["tuned_xcd_order"], "candidates_timed": s["tuned_trials"]},
+        "stage_ms": s["stage_ms"],
+    }
+
+
 def cpu_baseline(rep, cl, rows: int, threads: int) -> dict:
     import numpy as np
 
@@ -85,6 +214,12 @@ def main() -> int:
     ap.add_argument("--cpu-rows", type=int, default=200_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--train-pairs", type=int, default=1_000_000)
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--c4-rows", type=int, default=2_000_000, help="0 skips the 2M-row learned comparison")
+    ap.add_argument("--c3-reps", type=int, default=512)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -184,6 +319,10 @@ def main() -> int:
             "algorithmic_ops": ops,
         },
     }
+    if not args.no_learned:
+        out["learned"] = learned_section(args, cl, rep, ms, run.result())
+    if not args.no_batched:
+        out["batched"] = batched_section(args)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_rows, n)} rows, {threads} threads")

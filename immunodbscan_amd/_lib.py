@@ -128,7 +128,7 @@ class IdbStats(C.Structure):
         ("tuned_xcd_order", C.c_uint32),
         ("tuned_trials", C.c_uint32),
         ("graph_used", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("encoder_ms", C.c_float),
     ]
 
     def as_dict(self) -> dict:

@@ -884,6 +884,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
          idb_outputs *o, uint32_t *n_uniques_h, idb_stats *st, const Extra *x = nullptr)
 {
     const bool learned = x && x->enc;
+    float encoder_ms = 0.f;
     const uint32_t N = d->n_rows;
     const uint32_t vw = d->v_width, jw = d->j_width;
     if (o->mem != IDB_MEM_HOST && o->mem != IDB_MEM_DEVICE) fail(c, IDB_ERR_ARG, "outputs.mem must be host or device");
@@ -1122,8 +1123,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         if (learned) {
             // embeddings of every row, then proposals verified by Myers
             uint16_t *zb = buf<uint16_t>(c, S_ZB, (size_t)N * 64);
+            check(c, hipEventRecord(c->kev[2], s), "event record");
             if (idb_encoder_embed_rows(x->enc, seq, off, len, N, zb, s) != hipSuccess)
                 fail(c, IDB_ERR_HIP, std::string("encoder: ") + idb_encoder_error(x->enc));
+            check(c, hipEventRecord(c->kev[3], s), "event record");
+            check(c, hipEventSynchronize(c->kev[3]), "event sync");
+            check(c, hipEventElapsedTime(&encoder_ms, c->kev[2], c->kev[3]), "event time");
             a.zb = zb;
             a.min_sim = x->min_sim;
         }
@@ -1312,6 +1317,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             for (const auto &kv : c->tuned)
                 if (kv.second.wpb == a.tune_wpb && kv.second.grid_mul == a.tune_grid_mul) st->tuned_trials = kv.second.trials;
         st->graph_used = graph ? 1u : 0u;
+        st->encoder_ms = encoder_ms;
         if (E > 0 && T > 0 && !graph) {
             (void)hipEventElapsedTime(&st->count_kernel_ms, c->kev[0], c->kev[1]);
             (void)hipEventElapsedTime(&st->fuse_kernel_ms, c->kev[2], c->kev[3]);

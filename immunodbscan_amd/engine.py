@@ -85,7 +85,7 @@ class DeviceRun:
     path without host copies.  Buffers come from immunodbscan_amd.hip."""
 
     def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, batch: bool = False,
-                 flags: int = 0):
+                 flags: int = 0, encoder=None, min_sim: float = 0.0):
         from .hip import DeviceBuffer, synchronize
 
         self.ctx = ctx or default_context()
@@ -108,10 +108,15 @@ class DeviceRun:
         self.starts = np.ascontiguousarray(rep.rep_start, dtype=np.uint32)
         self.nu = np.zeros(max(len(self.starts) - 1, 1), np.uint32)
         self.stats = _lib.IdbStats()
+        self.encoder = encoder
+        self.min_sim = float(min_sim)
         synchronize()
 
     def __call__(self) -> _lib.IdbStats:
-        if self.batch:
+        if self.encoder is not None:
+            rc = _lib.lib.idb_dbscan_learned(self.ctx.handle, self.encoder.handle, self.min_sim, self.ds, self.prm,
+                                             self.outs, self.stats)
+        elif self.batch:
             rc = _lib.lib.idb_dbscan_batch(self.ctx.handle, self.ds, self.starts.ctypes.data, len(self.starts) - 1,
                                            self.prm, self.outs, self.nu.ctypes.data, self.stats)
         else:

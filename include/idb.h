@@ -112,7 +112,7 @@ typedef struct {
     float stage_ms[IDB_N_STAGES]; /* with IDB_FLAG_GRAPH the graph's time is the core-point stage's */
     uint32_t tuned_wpb, tuned_grid_mul, tuned_xcd_order, tuned_trials;  /* count-kernel launch shape */
     uint32_t graph_used;          /* 1: the tail ran as one graph launch             */
-    uint32_t reserved;
+    float encoder_ms;             /* learned filter: embedding of every row (HIP events) */
 } idb_stats;
 
 /* Context on HIP device `device`: a stream and a reusable device arena.
