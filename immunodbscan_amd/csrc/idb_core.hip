@@ -42,7 +42,7 @@ enum Slot {
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
-    S_DIST, S_CNT2,
+    S_DIST, S_CNT2, S_EQG,
     S_COUNT
 };
 
@@ -410,9 +410,11 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const int32_t *jb, uint32_t jw, uint32_t single, uint32_t gj, uint32_t gvj,
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
                                const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len,
-                               uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag)
+                               uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag,
+                               uint32_t *eqg)
 {
     __shared__ uint8_t cm[256];
+    __shared__ uint32_t qgc[256 * IDB_QGRAM_WORDS];  // one 64-byte 3-gram count row per thread
     for (uint32_t b = threadIdx.x; b < 256; b += blockDim.x) cm[b] = cmap[b];
     __syncthreads();
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
@@ -474,6 +476,24 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                 }
                 dst[k0 / 16] = make_uint4(w[0], w[1], w[2], w[3]);
             }
+        }
+        if (eqg) {
+            // 3-gram profile (codes 0..3): counts of c[i-2] c[i-1] c[i],
+            // saturating at 255
+            uint32_t *row = qgc + threadIdx.x * IDB_QGRAM_WORDS;
+            uint8_t *cnt8 = (uint8_t *)row;
+            for (uint32_t w = 0; w < IDB_QGRAM_WORDS; ++w) row[w] = 0u;
+            uint32_t hist = 0;
+            for (uint32_t k0 = 0; k0 < L; k0 += 16) {
+                const uint4 v = row16(s, L, k0);
+                for (uint32_t b = 0; b < 16 && k0 + b < L; ++b) {
+                    hist = ((hist << 2) | (cm[byte_of(v, b)] & 3u)) & 63u;
+                    if (k0 + b >= 2 && cnt8[hist] < 255) ++cnt8[hist];
+                }
+            }
+            uint4 *dst = (uint4 *)(eqg + (size_t)e * IDB_QGRAM_WORDS);
+            for (uint32_t w = 0; w < IDB_QGRAM_WORDS / 4; ++w)
+                dst[w] = make_uint4(row[4 * w], row[4 * w + 1], row[4 * w + 2], row[4 * w + 3]);
         }
         if (epk) {
             // 16 letters per step: one 2-bit word or four 8-bit words
@@ -953,7 +973,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     } else {
         W = std::max(1u, (maxlen + 31) / 32);
         if (W > IDB_MAX_W) fail(c, IDB_ERR_UNSUPPORTED, "Levenshtein rows longer than 256 letters are not supported");
-        if (idb_myers_lds_per_wave(std::max(std::min(K + 1, 256u), 1u), W) > idb_lds_limit())
+        if (idb_myers_lds_per_wave(std::max(std::min(K + 1, 256u), 1u), W) +
+                (K <= 4 ? idb_qgram_lds_per_wave() : 0) > idb_lds_limit())
             fail(c, IDB_ERR_UNSUPPORTED, "alphabet of " + std::to_string(K) + " letters with rows of " +
                                              std::to_string(maxlen) + " letters exceeds the LDS budget");
         S = std::max(16u, (maxlen + 15) / 16 * 16);
@@ -1069,9 +1090,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // The absent-letter filter pays off where random letter matches are
         // rare (protein alphabets); with 4 letters nearly every pair passes.
         const bool filt = !ham && W == 1 && K >= 8 && K <= 31;
+        // Nucleotide rows: the 3-gram profile bound (learned runs use their own proposals).
+        const bool qgram = !ham && !learned && K <= 4 && maxlen >= 16;
+        uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
-                           ham ? 1u : 0u, elen, ent, ecodes, epk, bflag);
+                           ham ? 1u : 0u, elen, ent, ecodes, epk, bflag, eqg);
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
         uint32_t *qbcnt = buf<uint32_t>(c, S_QBCNT, (size_t)E + 1);
@@ -1107,13 +1131,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.gj = gj;
         a.single_bucket = single;
         a.band_filter = filt ? 1u : 0u;
+        a.qgram_filter = qgram ? 1u : 0u;
+        a.e_qg = eqg;
         a.cnt = cnt;
         a.work = work;
         // Neighbour pairs found by the count pass are kept for the fuse step;
         // if they overflow the list, the fuse pass re-evaluates the tiles.
         // Waves own chunks of 256 pairs (64 for small requested capacities).
-        edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 8ull * E);
-        edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 29);
+        edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 64ull * E);
+        edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 31);
         a.chunk_log2 = edge_cap >= (1ull << 20) ? 8u : 6u;
         a.max_chunks = (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
         edge_cap = (unsigned long long)a.max_chunks << a.chunk_log2;
@@ -1307,7 +1333,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->gated_letters = wk[3];
         st->myers_pairs = wk[5];
         st->myers_letters = wk[6];
-        st->band_filter = a.band_filter;
+        st->band_filter = a.band_filter ? 1u : (a.qgram_filter ? 2u : 0u);
         st->fuse_from_edges = from_edges ? 1u : 0u;
         st->tuned_wpb = a.tune_wpb;
         st->tuned_grid_mul = a.tune_grid_mul;

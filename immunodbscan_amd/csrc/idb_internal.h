@@ -12,7 +12,8 @@
 //             contiguous run.  Per entry a 32-byte record (idb_entry, loaded
 //             in one round trip) and the row's letters as dense 8-bit codes
 //             (stride S bytes, S a multiple of 16) or, for Hamming mode,
-//             packed words (stride HW words).
+//             packed words (stride HW words); for nucleotide rows a 3-gram
+//             profile (64 saturating byte counts) per entry.
 //   tiles:    (query block, target block) pairs of <= 64 entries each inside a
 //             group (a bucket; in Hamming mode a bucket's run of one length)
 //             whose length band can hold a neighbour pair.
@@ -31,6 +32,7 @@
 #define IDB_MAX_HW 64             // Hamming packed words per row
 #define IDB_MAX_LEN 65535u
 #define IDB_ENC_P 32              // letters per row the encoder reads
+#define IDB_QGRAM_WORDS 16        // 3-gram profile: 64 byte counts per row
 
 struct idb_tile {
     uint32_t q0;                  // first query entry
@@ -67,6 +69,8 @@ struct idb_pair_args {
     uint32_t gj;                  // J genes per bucket row (32*jw)
     uint32_t single_bucket;       // 1: every pair is gated explicitly
     uint32_t band_filter;         // 1: absent-letter filter before Myers (alphabets <= 31 letters)
+    uint32_t qgram_filter;        // 1: 3-gram profile filter before Myers (alphabets <= 4 letters)
+    const uint32_t *e_qg;         // 3-gram profiles, IDB_QGRAM_WORDS per entry
     int32_t *cnt;                 // count pass output
     const int32_t *core;          // fuse pass input (per row)
     uint32_t *par;                // fuse pass: union-find parents
@@ -93,6 +97,8 @@ __host__ __device__ inline size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
 {
     return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t);
 }
+// Extra LDS per wave for the query profiles of the 3-gram filter.
+__host__ __device__ inline size_t idb_qgram_lds_per_wave() { return (size_t)IDB_TILE * IDB_QGRAM_WORDS * sizeof(uint32_t); }
 // The device's LDS limit per block (opt-in value where the device has one).
 size_t idb_lds_limit();
 

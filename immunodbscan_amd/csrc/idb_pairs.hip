@@ -298,17 +298,27 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
 }
 
-// FILTER (alphabets of at most 31 letters): before Myers, a pair must pass the
-// absent-letter bound, a few bit operations per pair; surviving pairs are
-// queued in LDS and verified by Myers 64 at a time.
-template <int W, bool FUSE, bool FILTER>
+// FILT selects an exact lower bound that a pair must pass before Myers;
+// surviving pairs are queued in LDS and verified by Myers 64 at a time.
+//   1 (alphabets of 8-31 letters): the absent-letter bound, a few bit
+//     operations per pair;
+//   2 (alphabets of at most 4 letters, longer rows): the 3-gram profile bound
+//     of E. Ukkonen ("Approximate string-matching with q-grams and maximal
+//     matches", TCS 1992): an edit changes at most q q-grams on either side,
+//     so lev(x, y) >= |P_x - P_y|_1 / (2 q).  Profiles are 64 saturating byte
+//     counts per row (saturation only lowers the L1 distance, so the bound
+//     stays a lower bound); the L1 distance is 16 v_sad_u8 per pair, query
+//     profiles are read from LDS by broadcast.
+template <int W, bool FUSE, int FILT>
 __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = threadIdx.x >> 6;
     const uint32_t K = a.K;
-    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
+    constexpr uint32_t QG_WORDS = FILT == 2 ? IDB_TILE * IDB_QGRAM_WORDS : 0u;
+    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t) + QG_WORDS);
+    uint32_t *qg = peq + idb_myers_lds_per_wave(K, W) / sizeof(uint32_t);
     EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
     uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
     int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
@@ -358,10 +368,29 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
         int32_t acc_y = 0;
-        uint32_t qlen = 0;  // queued pairs (FILTER)
+        uint32_t qlen = 0;  // queued pairs (FILT)
         if (!FUSE) {
             qacc[lane] = 0;
             tacc[lane] = 0;
+        }
+        uint32_t tq[FILT == 2 ? IDB_QGRAM_WORDS : 1];
+        if (FILT == 2) {
+            const uint4 *src = (const uint4 *)(a.e_qg + (size_t)p * IDB_QGRAM_WORDS);
+#pragma unroll
+            for (int k = 0; k < (int)IDB_QGRAM_WORDS / 4; ++k) {
+                const uint4 v = tvalid ? src[k] : make_uint4(0, 0, 0, 0);
+                tq[4 * k] = v.x;
+                tq[4 * k + 1] = v.y;
+                tq[4 * k + 2] = v.z;
+                tq[4 * k + 3] = v.w;
+            }
+            if (lane < qn) {
+                const uint4 *qs = (const uint4 *)(a.e_qg + (size_t)(tile.q0 + lane) * IDB_QGRAM_WORDS);
+                uint4 *qd = (uint4 *)(qg + lane * IDB_QGRAM_WORDS);
+#pragma unroll
+                for (int k = 0; k < (int)IDB_QGRAM_WORDS / 4; ++k) qd[k] = qs[k];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         }
 
         for (uint32_t qi = 0; qi < qn; ++qi) {
@@ -383,16 +412,35 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             gated += __popcll(vm);
             gletters += (unsigned long long)__popcll(vm) * lx;
 
-            if (FILTER) {
-                // Absent-letter bound: a letter of x whose symbol never occurs
-                // in y must be substituted or deleted, one edit per position,
-                // so lev(x, y) >= #{i : x_i not in y} (and symmetrically).
-                // The planes hold the letters seen at least once / twice.
-                const uint32_t p1x = rl(qe.pres1, qi), p2x = rl(qe.pres2, qi);
-                const uint32_t lbx = __builtin_popcount(p1x & ~te.pres1) + __builtin_popcount(p2x & ~te.pres1);
-                const uint32_t lby = __builtin_popcount(te.pres1 & ~p1x) + __builtin_popcount(te.pres2 & ~p1x);
-                const uint32_t lb = lbx > lby ? lbx : lby;
-                const bool pass = valid && (int32_t)lb <= t;
+            if (FILT) {
+                bool pass;
+                if (FILT == 1) {
+                    // Absent-letter bound: a letter of x whose symbol never
+                    // occurs in y must be substituted or deleted, one edit per
+                    // position, so lev(x, y) >= #{i : x_i not in y} (and
+                    // symmetrically).  The planes hold the letters seen at
+                    // least once / twice.
+                    const uint32_t p1x = rl(qe.pres1, qi), p2x = rl(qe.pres2, qi);
+                    const uint32_t lbx = __builtin_popcount(p1x & ~te.pres1) + __builtin_popcount(p2x & ~te.pres1);
+                    const uint32_t lby = __builtin_popcount(te.pres1 & ~p1x) + __builtin_popcount(te.pres2 & ~p1x);
+                    const uint32_t lb = lbx > lby ? lbx : lby;
+                    pass = valid && (int32_t)lb <= t;
+                } else {
+                    // half the profile first: its L1 distance is already a
+                    // lower bound, and most pairs fail it
+                    const uint4 *qv = (const uint4 *)(qg + qi * IDB_QGRAM_WORDS);
+                    uint32_t l1 = 0;
+#pragma unroll
+                    for (int k = 0; k < (int)IDB_QGRAM_WORDS / 4; ++k) {
+                        if (k == (int)IDB_QGRAM_WORDS / 8 && !__ballot(valid && (int32_t)l1 <= 6 * t)) break;
+                        const uint4 v = qv[k];
+                        l1 = __builtin_amdgcn_sad_u8(v.x, tq[4 * k], l1);
+                        l1 = __builtin_amdgcn_sad_u8(v.y, tq[4 * k + 1], l1);
+                        l1 = __builtin_amdgcn_sad_u8(v.z, tq[4 * k + 2], l1);
+                        l1 = __builtin_amdgcn_sad_u8(v.w, tq[4 * k + 3], l1);
+                    }
+                    pass = valid && (int32_t)l1 <= 6 * t;
+                }
                 const unsigned long long pm = __ballot(pass);
                 if (!pm) continue;
                 const uint32_t below =
@@ -448,7 +496,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 else atomicMin(a.bmin + x, y);
             }
         }
-        if (FILTER && qlen) {
+        if (FILT && qlen) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
                                  mletters);
@@ -913,24 +961,40 @@ uint32_t pick_wpb(const idb_pair_args &a, uint32_t most)
     return std::max(1u, std::min(w, most));
 }
 
-template <int W, bool FUSE, bool FILTER>
+template <int W, bool FUSE, int FILT>
 hipError_t launch_myers_w(const idb_pair_args &a, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
 {
-    const void *k = (const void *)k_pairs_myers<W, FUSE, FILTER>;
+    const void *k = (const void *)k_pairs_myers<W, FUSE, FILT>;
     if (shm > 65536) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
     const dim3 grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm), a.tune_grid_mul));
-    hipLaunchKernelGGL((k_pairs_myers<W, FUSE, FILTER>), grid, block, shm, s, a, wpb);
+    hipLaunchKernelGGL((k_pairs_myers<W, FUSE, FILT>), grid, block, shm, s, a, wpb);
     return hipGetLastError();
+}
+
+template <bool FUSE, int FILT>
+hipError_t launch_myers_f(const idb_pair_args &a, uint32_t W, dim3 block, size_t shm, uint32_t wpb, hipStream_t s)
+{
+    switch (W) {
+    case 1: return launch_myers_w<1, FUSE, FILT>(a, block, shm, wpb, s);
+    case 2: return launch_myers_w<2, FUSE, FILT>(a, block, shm, wpb, s);
+    case 3: return launch_myers_w<3, FUSE, FILT>(a, block, shm, wpb, s);
+    case 4: return launch_myers_w<4, FUSE, FILT>(a, block, shm, wpb, s);
+    case 5: return launch_myers_w<5, FUSE, FILT>(a, block, shm, wpb, s);
+    case 6: return launch_myers_w<6, FUSE, FILT>(a, block, shm, wpb, s);
+    case 7: return launch_myers_w<7, FUSE, FILT>(a, block, shm, wpb, s);
+    case 8: return launch_myers_w<8, FUSE, FILT>(a, block, shm, wpb, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <bool FUSE>
 hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
 {
     if (a.n_tiles == 0) return hipSuccess;
-    const size_t per_wave = idb_myers_lds_per_wave(a.K, W);
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, W) + (a.qgram_filter ? idb_qgram_lds_per_wave() : 0);
     const size_t limit = idb_lds_limit();
     uint32_t wpb = (uint32_t)(std::min<size_t>(limit, 65536) / per_wave);
     if (wpb > 4) wpb = 4;
@@ -939,18 +1003,9 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
     wpb = pick_wpb(a, wpb);
     const dim3 block(64 * wpb);
     const size_t shm = per_wave * wpb;
-    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, true>(a, block, shm, wpb, s);
-    switch (W) {
-    case 1: return launch_myers_w<1, FUSE, false>(a, block, shm, wpb, s);
-    case 2: return launch_myers_w<2, FUSE, false>(a, block, shm, wpb, s);
-    case 3: return launch_myers_w<3, FUSE, false>(a, block, shm, wpb, s);
-    case 4: return launch_myers_w<4, FUSE, false>(a, block, shm, wpb, s);
-    case 5: return launch_myers_w<5, FUSE, false>(a, block, shm, wpb, s);
-    case 6: return launch_myers_w<6, FUSE, false>(a, block, shm, wpb, s);
-    case 7: return launch_myers_w<7, FUSE, false>(a, block, shm, wpb, s);
-    case 8: return launch_myers_w<8, FUSE, false>(a, block, shm, wpb, s);
-    default: return hipErrorInvalidValue;
-    }
+    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, 1>(a, block, shm, wpb, s);
+    if (a.qgram_filter) return launch_myers_f<FUSE, 2>(a, W, block, shm, wpb, s);
+    return launch_myers_f<FUSE, 0>(a, W, block, shm, wpb, s);
 }
 
 template <int HWMAX, int BITS, bool FUSE>

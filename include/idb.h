@@ -98,7 +98,7 @@ typedef struct {
     uint32_t n_rows, n_uniques, n_cores, n_clusters, n_noise;
     uint32_t max_len, alphabet_size, n_buckets;
     uint32_t fuse_from_edges;     /* 1: clusters grown from the count pass's pair list       */
-    uint32_t band_filter;         /* 1: band-match filter ran before Myers                   */
+    uint32_t band_filter;         /* exact bound before Myers: 1 absent letters, 2 3-grams   */
     uint64_t bucket_entries, tiles;
     uint64_t gated_pairs;         /* pairs reaching a distance kernel (count pass)   */
     uint64_t neighbour_pairs;     /* unordered unique pairs with N(x,y)              */
