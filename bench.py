@@ -24,7 +24,11 @@ Beside the headline exact line the JSON carries:
             ARI and the speedup over the exact path are reported;
   batched   512 repertoires (about 8.6M nucleotide rows, Hamming, eps 0.15,
             minPts 2) in one batched call with the autotuned, graph-captured
-            tail; pairs/s is the sum of every repertoire's N(N-1)/2.
+            tail; pairs/s is the sum of every repertoire's N(N-1)/2;
+  large     one repertoire of 20,000,000 nucleotide CDR3s (24-150 nt,
+            100k seeds, 3% substitutions, 0.5% indels), Levenshtein t = 3
+            edits + V/J gate, minPts 3: the exact path with the 3-gram bound
+            and 5-word Myers (the learned filter reads at most 32 letters).
 """
 from __future__ import annotations
 
@@ -54,14 +58,19 @@ def myers_ops_per_letter(W: int) -> int:
 # absent-letter filter per pair: per direction two AND-NOTs of the presence
 # planes and two bit counts, then a maximum and the threshold compare
 FILTER_OPS_PER_PAIR = 10
+# 3-gram profile bound per pair: 16 four-byte absolute-difference sums, the
+# scaled threshold and the compare
+QGRAM_OPS_PER_PAIR = 18
 
 
 def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
     W = max(1, (s["max_len"] + 31) // 32)
     gate_ops = 2 * (vw + jw) + 2
     ops = s["gated_pairs"] * gate_ops + s["myers_letters"] * myers_ops_per_letter(W)
-    if s["band_filter"]:
+    if s["band_filter"] == 1:
         ops += s["gated_pairs"] * FILTER_OPS_PER_PAIR
+    elif s["band_filter"] == 2:
+        ops += s["gated_pairs"] * QGRAM_OPS_PER_PAIR
     return ops
 
 
@@ -182,6 +191,49 @@ def batched_section(args) -> dict:
     }
 
 
+def pmc_traffic(rows: int):
+    """HBM bytes per launch of the count kernel from the committed PMC
+    profile of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by a 256 MiB device copy measured in the same run); None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "c2_count_kernel_pmc.json")
+    if rows != 200_000 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["traffic_bytes"])
+
+
+def large_section(args) -> dict:
+    from immunodbscan_amd import configs, engine, synth
+
+    rep = synth.generate(configs.c5_params(n_rows=args.c5_rows, n_seeds=max(1, args.c5_rows // 200)))
+    log(f"large: generated {rep.n} rows")
+    run = engine.DeviceRun(rep, configs.C5)
+    ms, st = timed(run, 2, 1)
+    s = st.as_dict()
+    n = rep.n
+    W = max(1, (s["max_len"] + 31) // 32)
+    ops = count_kernel_ops(s, rep.vw, rep.jw)
+    kms = s["count_kernel_ms"]
+    log(f"large: {ms:.1f} ms/step")
+    return {
+        "workload": f"{n} nt CDR3s, 24-150 nt, t=3 edits + V/J gate, minPts=3",
+        "rows": n,
+        "ms_per_step": ms,
+        "pairs_per_sec": n * (n - 1) / 2 / (ms * 1e-3),
+        "seqs_clustered_per_sec": n / (ms * 1e-3),
+        "gated_pairs": s["gated_pairs"],
+        "myers_pairs": s["myers_pairs"],
+        "neighbour_pairs": s["neighbour_pairs"],
+        "clusters": s["n_clusters"],
+        "myers_words": W,
+        "filter": "3-gram profile bound" if s["band_filter"] == 2 else s["band_filter"],
+        "stage_ms": s["stage_ms"],
+        "count_kernel_roofline": {"bound": "valu", "achieved": ops / (kms * 1e-3) / 1e12, "peak": VALU_PEAK_TOPS,
+                                  "unit": "TOP/s", "frac": ops / (kms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
+                                  "kernel_ms": kms, "algorithmic_ops": ops},
+    }
+
+
 def cpu_baseline(rep, cl, rows: int, threads: int) -> dict:
     import numpy as np
 
@@ -220,6 +272,7 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--c4-rows", type=int, default=2_000_000, help="0 skips the 2M-row learned comparison")
     ap.add_argument("--c3-reps", type=int, default=512)
+    ap.add_argument("--c5-rows", type=int, default=20_000_000, help="0 skips the 20M-row nucleotide line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -314,7 +367,7 @@ def main() -> int:
             "peak": VALU_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": achieved / VALU_PEAK_TOPS,
-            "traffic": None,
+            "traffic": pmc_traffic(n),
             "kernel_ms": kms,
             "algorithmic_ops": ops,
         },
@@ -323,6 +376,8 @@ def main() -> int:
         out["learned"] = learned_section(args, cl, rep, ms, run.result())
     if not args.no_batched:
         out["batched"] = batched_section(args)
+    if args.c5_rows:
+        out["large"] = large_section(args)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_rows, n)} rows, {threads} threads")
