@@ -61,32 +61,37 @@ __device__ __forceinline__ float gelu_grad(float x)
 
 // ---------------------------------------------------------------------------
 // GEMM: out[m][n] = sum_k A[m][k] B[n][k] over a block tile of 128 x 128 with
-// k chunks of 32, double-buffered in LDS.  Four waves own 64 x 64 each (2 x 2
-// MFMA tiles of 32 x 32).  LDS rows hold 32 k values plus 8 of padding (80
-// bytes), so the 16-byte fragment reads of 16 consecutive rows hit disjoint
-// banks.  Loaders fetch 8 consecutive k of one row (ROW) or 8 consecutive rows
-// at one k (COL, stored transposed) per 16-byte load.
-constexpr uint32_t GM = 128, GN = 128, GK = 32, LDK = GK + 8;
+// k chunks of 64, double-buffered in LDS (the next chunk's global loads are in
+// flight while the current chunk's 16 MFMAs per wave run).  Four waves own
+// 64 x 64 each (2 x 2 MFMA tiles of 32 x 32).  LDS rows hold 64 k values plus
+// 8 of padding (144 bytes), so the 16-byte fragment reads of 16 consecutive
+// rows hit disjoint banks.  Loaders fetch 8 consecutive k of one row (ROW) or
+// 8 consecutive rows at one k (COL, stored transposed) per 16-byte load.
+constexpr uint32_t GM = 128, GN = 128, GK = 64, LDK = GK + 8;
+constexpr uint32_t GCH = GM * GK / 8 / 256;  // 16-byte chunks per thread and operand
 
+// k at or past kend (the end of this block's k range) reads as zero.
 template <bool COL, class L>
-__device__ __forceinline__ void fetch_tile(const L &ld, uint32_t r0, uint32_t k0, uint32_t tid, uint4 (&v)[2])
+__device__ __forceinline__ void fetch_tile(const L &ld, uint32_t r0, uint32_t k0, uint32_t kend, uint32_t tid,
+                                           uint4 (&v)[GCH])
 {
 #pragma unroll
-    for (uint32_t i = 0; i < 2; ++i) {
+    for (uint32_t i = 0; i < GCH; ++i) {
         const uint32_t c = tid + i * 256;
-        if (!COL) v[i] = ld.fetch(r0 + (c >> 2), k0 + (c & 3) * 8);
-        else v[i] = ld.fetch(r0 + (c & 15) * 8, k0 + (c >> 4));
+        const uint32_t k = COL ? k0 + (c >> 4) : k0 + (c & 7) * 8;
+        v[i] = make_uint4(0, 0, 0, 0);
+        if (k < kend) v[i] = COL ? ld.fetch(r0 + (c & 15) * 8, k) : ld.fetch(r0 + (c >> 3), k);
     }
 }
 
 template <bool COL>
-__device__ __forceinline__ void store_tile(uint16_t *lds, const uint4 (&v)[2], uint32_t tid)
+__device__ __forceinline__ void store_tile(uint16_t *lds, const uint4 (&v)[GCH], uint32_t tid)
 {
 #pragma unroll
-    for (uint32_t i = 0; i < 2; ++i) {
+    for (uint32_t i = 0; i < GCH; ++i) {
         const uint32_t c = tid + i * 256;
         if (!COL) {
-            *(uint4 *)(lds + (c >> 2) * LDK + (c & 3) * 8) = v[i];
+            *(uint4 *)(lds + (c >> 3) * LDK + (c & 7) * 8) = v[i];
         } else {
             const uint32_t r = (c & 15) * 8, k = c >> 4;
             const uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
@@ -97,7 +102,7 @@ __device__ __forceinline__ void store_tile(uint16_t *lds, const uint4 (&v)[2], u
 }
 
 template <class LA, class LB, class EP>
-__global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, EP ep, uint32_t M, uint32_t N, uint32_t K, uint32_t ksplit)
+__global__ __launch_bounds__(256, 2) void k_gemm(LA la, LB lb, EP ep, uint32_t M, uint32_t N, uint32_t K, uint32_t ksplit)
 {
     __shared__ __align__(16) uint16_t As[2][GM * LDK];
     __shared__ __align__(16) uint16_t Bs[2][GN * LDK];
@@ -113,9 +118,9 @@ __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, EP ep, uint32_t M, u
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
     if (kb >= ke) return;
-    uint4 ra[2], rb[2];
-    fetch_tile<LA::COL>(la, m0, kb, tid, ra);
-    fetch_tile<LB::COL>(lb, n0, kb, tid, rb);
+    uint4 ra[GCH], rb[GCH];
+    fetch_tile<LA::COL>(la, m0, kb, ke, tid, ra);
+    fetch_tile<LB::COL>(lb, n0, kb, ke, tid, rb);
     store_tile<LA::COL>(As[0], ra, tid);
     store_tile<LB::COL>(Bs[0], rb, tid);
     __syncthreads();
@@ -123,8 +128,8 @@ __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, EP ep, uint32_t M, u
     for (uint32_t k0 = kb; k0 < ke; k0 += GK) {
         const bool more = k0 + GK < ke;
         if (more) {
-            fetch_tile<LA::COL>(la, m0, k0 + GK, tid, ra);
-            fetch_tile<LB::COL>(lb, n0, k0 + GK, tid, rb);
+            fetch_tile<LA::COL>(la, m0, k0 + GK, ke, tid, ra);
+            fetch_tile<LB::COL>(lb, n0, k0 + GK, ke, tid, rb);
         }
 #pragma unroll
         for (uint32_t ks = 0; ks < GK / 16; ++ks) {
@@ -168,12 +173,13 @@ __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, EP ep, uint32_t M, u
 struct LdConv {
     static constexpr bool COL = false;
     const uint16_t *X;
-    uint32_t cin, M;
+    uint32_t cin, M;  // cin a power of two
     int flip;
     __device__ uint4 fetch(uint32_t m, uint32_t k) const
     {
         if (m >= M) return make_uint4(0, 0, 0, 0);
-        const int t = (int)(k / cin), c = (int)(k % cin), p = (int)(m & 31u);
+        const uint32_t sh = 31u - __builtin_clz(cin);
+        const int t = (int)(k >> sh), c = (int)(k & (cin - 1u)), p = (int)(m & 31u);
         const int q = flip ? p - t + 2 : p + t - 2;
         if (q < 0 || q >= (int)P) return make_uint4(0, 0, 0, 0);
         return *(const uint4 *)(X + ((size_t)(m & ~31u) + q) * cin + c);
@@ -210,11 +216,12 @@ struct LdGradT {
 struct LdConvT {
     static constexpr bool COL = true;
     const uint16_t *X;
-    uint32_t cin, M;
+    uint32_t cin, M;  // cin a power of two
     __device__ uint4 fetch(uint32_t kc, uint32_t m) const
     {
         if (m >= M || kc >= T * cin) return make_uint4(0, 0, 0, 0);
-        const int t = (int)(kc / cin), c = (int)(kc % cin), q = (int)(m & 31u) + t - 2;
+        const uint32_t sh = 31u - __builtin_clz(cin);
+        const int t = (int)(kc >> sh), c = (int)(kc & (cin - 1u)), q = (int)(m & 31u) + t - 2;
         if (q < 0 || q >= (int)P) return make_uint4(0, 0, 0, 0);
         return *(const uint4 *)(X + ((size_t)(m & ~31u) + q) * cin + c);
     }
@@ -644,7 +651,7 @@ void backward(idb_encoder *e, uint32_t nseq)
         EpAtomic ep{e->grad + offs[l], T * cin};
         hipLaunchKernelGGL((k_gemm<LdGradT, LdConvT, EpAtomic>), dim3(1, (T * cin + GN - 1) / GN, nsplit), dim3(256), 0,
                            s, la, lb, ep, C, T * cin, M, ks);
-        hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(C), 0, s, dH, M, e->grad + boff[l]);
+        hipLaunchKernelGGL(k_bias_grad, dim3(2048), dim3(C), 0, s, dH, M, e->grad + boff[l]);
         LdConv la2{dH, C, M, 1};
         LdRows lb2{e->wt[l], T * C, cin};
         EpConvBwd ep2{l ? e->H[l - 1] : nullptr, e->blen, cin, l ? dHn : nullptr, l ? nullptr : e->dX0};
