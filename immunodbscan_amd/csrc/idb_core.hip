@@ -57,7 +57,7 @@ struct idb_ctx {
     hipEvent_t kev[4] = {};       // around the count and the fuse kernels
     // autotuner: best count-kernel launch shape per kernel and shape class
     struct Tuned {
-        uint32_t wpb, grid_mul, no_xcd, trials;
+        uint32_t wpb, grid_mul, no_xcd, trials, dyn;
     };
     std::map<std::array<uint32_t, 6>, Tuned> tuned;
     hipGraphExec_t gexec = nullptr;  // the tail graph, updated in place
@@ -340,11 +340,11 @@ __device__ __forceinline__ uint32_t popc_words(const int32_t *w, uint32_t n)
 }
 
 __global__ void k_membership(const uint32_t *uniq, const uint32_t *num, const int32_t *vb, uint32_t vw,
-                             const int32_t *jb, uint32_t jw, uint32_t single, uint64_t *memb)
+                             const int32_t *jb, uint32_t jw, uint32_t single, uint32_t cap, uint64_t *memb)
 {
     const uint32_t U = num[0];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= U; k += gridDim.x * blockDim.x) {
-        if (k == U) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= cap; k += gridDim.x * blockDim.x) {
+        if (k >= U) {
             memb[k] = 0u;
             continue;
         }
@@ -353,10 +353,11 @@ __global__ void k_membership(const uint32_t *uniq, const uint32_t *num, const in
     }
 }
 
-// One entry per (unique row, V gene, J gene); key = bucket << 16 | length.
+// One entry per (unique row, V gene, J gene); key = bucket << lb | length
+// (lb = bits of the longest row).
 __global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const uint64_t *eoff, const int32_t *vb,
                                uint32_t vw, const int32_t *jb, uint32_t jw, const uint32_t *len, const uint32_t *rid,
-                               uint32_t single, uint32_t gj, uint32_t gvj, uint64_t *ekey, uint32_t *erow)
+                               uint32_t single, uint32_t gj, uint32_t gvj, uint32_t lb, uint64_t *ekey, uint32_t *erow)
 {
     const uint32_t U = num[0];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < U; k += gridDim.x * blockDim.x) {
@@ -365,7 +366,7 @@ __global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const 
         const uint64_t L = len[u];
         uint64_t o = eoff[k];
         if (single) {
-            ekey[o] = (r << 16) | L;
+            ekey[o] = (r << lb) | L;
             erow[o] = u;
             continue;
         }
@@ -380,7 +381,7 @@ __global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const 
                         const uint32_t j = wj * 32 + __builtin_ctz(mj);
                         mj &= mj - 1;
                         const uint64_t bucket = r * gvj + (uint64_t)v * gj + j;
-                        ekey[o] = (bucket << 16) | L;
+                        ekey[o] = (bucket << lb) | L;
                         erow[o] = u;
                         ++o;
                     }
@@ -409,7 +410,7 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const uint64_t *off, const uint32_t *copies, const int32_t *vb, uint32_t vw,
                                const int32_t *jb, uint32_t jw, uint32_t single, uint32_t gj, uint32_t gvj,
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
-                               const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len,
+                               const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len, uint32_t lb,
                                uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag,
                                uint32_t *eqg)
 {
@@ -420,11 +421,11 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
         const uint64_t k = ekey[e];
         const uint32_t u = erow[e];
-        const uint32_t L = (uint32_t)(k & 0xFFFFu);
-        const uint64_t bucket = k >> 16;
+        const uint32_t L = (uint32_t)(k & ((1ull << lb) - 1ull));
+        const uint64_t bucket = k >> lb;
         elen[e] = (uint16_t)L;
         // a group is a bucket, or in Hamming mode a bucket's run of one length
-        bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> 16) != bucket);
+        bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> lb) != bucket);
         idb_entry r;
         r.row = u;
         r.cp = copies[u];
@@ -518,12 +519,13 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
     }
 }
 
-// Query blocks per bucket.
+// Query blocks per bucket over all E + 1 slots (0 past the B buckets), so the
+// scan's length is known on the host without reading B back.
 __global__ void k_bucket_blocks(const uint32_t *bstart, const uint32_t *nb_dev, uint32_t E, uint32_t *qbcnt)
 {
     const uint32_t B = nb_dev[0];
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= B; b += gridDim.x * blockDim.x) {
-        if (b == B) {
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= E; b += gridDim.x * blockDim.x) {
+        if (b >= B) {
             qbcnt[b] = 0;
             continue;
         }
@@ -585,8 +587,8 @@ __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, con
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
-    for (uint32_t qb = blockIdx.x * blockDim.x + threadIdx.x; qb <= QB; qb += gridDim.x * blockDim.x) {
-        if (qb == QB) {
+    for (uint32_t qb = blockIdx.x * blockDim.x + threadIdx.x; qb <= E; qb += gridDim.x * blockDim.x) {
+        if (qb >= QB) {
             tcnt[qb] = 0;
             continue;
         }
@@ -596,9 +598,27 @@ __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, con
     }
 }
 
+// Sizes the host needs, gathered on the device so that one read-back brings
+// them: sz[0..1] = entries (eoff[U]); sz[2] = buckets, sz[3] = query blocks,
+// sz[4] = tiles.
+__global__ void k_size_entries(const uint64_t *eoff, const uint32_t *num, uint32_t *sz)
+{
+    const uint64_t e = eoff[num[0]];
+    sz[0] = (uint32_t)e;
+    sz[1] = (uint32_t)(e >> 32);
+}
+
+__global__ void k_size_tiles(const uint32_t *num, const uint32_t *qboff, const uint32_t *toff, uint32_t *sz)
+{
+    const uint32_t B = num[1], QB = qboff[B];
+    sz[2] = B;
+    sz[3] = QB;
+    sz[4] = toff[QB];
+}
+
 __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, const uint64_t *ekey,
-                             uint32_t gvj, const uint32_t *toff, idb_tile *tiles)
+                             uint32_t lb, uint32_t gvj, const uint32_t *toff, idb_tile *tiles)
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
@@ -608,7 +628,7 @@ __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, cons
         const uint32_t q0 = q.s + q.I * IDB_TILE;
         const uint32_t qn = min((uint32_t)IDB_TILE, q.e - q0);
         // bucket id within its repertoire: the canonical test needs only (v, j)
-        const uint32_t bucket = (uint32_t)((ekey[q.s] >> 16) % gvj);
+        const uint32_t bucket = (uint32_t)((ekey[q.s] >> lb) % gvj);
         uint32_t o = toff[qb];
         for (int64_t J = q.I; J <= jhi; ++J) {
             const uint32_t t0 = q.s + (uint32_t)J * IDB_TILE;
@@ -834,11 +854,12 @@ unsigned bits_for(uint64_t v)
 void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "event record"); }
 
 // Autotuner of the count kernel's launch shape: waves per block (1, 2, 4 as
-// LDS allows), grid of 1x or 2x the resident blocks, XCD-grouped or plain
-// block order.  Each candidate runs the real count pass on the real tiles
-// (counts restored from a snapshot in between) and the fastest by HIP events
-// is kept for this kernel and shape class (kernel, words, code width, filter,
-// alphabet, log2 tiles) for the context's lifetime.
+// LDS allows), grid of 1, 2, 4 or 8 times the resident blocks, XCD-grouped or
+// plain block order, and (Myers kernels) static or dynamic tile order.  Each
+// candidate runs the real count pass on the real tiles twice (counts restored
+// from a snapshot in between) and the fastest by HIP events is kept for this
+// kernel and shape class (kernel, words, code width, filter, alphabet, log2
+// tiles) for the context's lifetime.
 void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bool learned, uint32_t *small,
               unsigned long long *work)
 {
@@ -850,30 +871,46 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
         hipStream_t s = c->stream;
         int32_t *snap = buf<int32_t>(c, S_CNT2, N);
         check(c, hipMemcpyAsync(snap, a.cnt, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune snapshot");
-        idb_ctx::Tuned best{0, 0, 0, 0};
+        idb_ctx::Tuned best{0, 0, 0, 0, 0};
         float best_ms = 1e30f;
         uint32_t trials = 0;
+        // dynamic tile order only for the Myers count kernel
+        const uint32_t dyn_max = (!learned && !ham) ? 1u : 0u;
+        for (uint32_t dyn = 0; dyn <= dyn_max; ++dyn)
         for (uint32_t wpb : {1u, 2u, 4u})
-            for (uint32_t mul : {1u, 2u})
+            for (uint32_t mul : {1u, 2u, 4u, 8u})
                 for (uint32_t nx : {0u, 1u}) {
                     a.tune_wpb = wpb;
                     a.tune_grid_mul = mul;
                     a.no_xcd_order = nx;
+                    a.tile_ctr = dyn ? (uint32_t *)(work + 7) : nullptr;
                     check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
                     check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
                     check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
-                    check(c, hipEventRecord(c->kev[0], s), "event record");
-                    const hipError_t e = learned ? idb_launch_count_learned(a, s)
-                                                 : (ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s));
-                    check(c, e, "autotune launch");
-                    check(c, hipEventRecord(c->kev[1], s), "event record");
-                    check(c, hipEventSynchronize(c->kev[1]), "event sync");
-                    float ms = 0.f;
-                    check(c, hipEventElapsedTime(&ms, c->kev[0], c->kev[1]), "event time");
+                    // best of two runs per shape
+                    float ms = 1e30f;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        if (rep) {
+                            check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s),
+                                  "tune reset");
+                            check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
+                            check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+                        }
+                        check(c, hipEventRecord(c->kev[0], s), "event record");
+                        const hipError_t e = learned ? idb_launch_count_learned(a, s)
+                                                     : (ham ? idb_launch_count_hamming(a, s)
+                                                            : idb_launch_count_myers(a, W, s));
+                        check(c, e, "autotune launch");
+                        check(c, hipEventRecord(c->kev[1], s), "event record");
+                        check(c, hipEventSynchronize(c->kev[1]), "event sync");
+                        float t = 0.f;
+                        check(c, hipEventElapsedTime(&t, c->kev[0], c->kev[1]), "event time");
+                        ms = std::min(ms, t);
+                    }
                     ++trials;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = {wpb, mul, nx, 0};
+                        best = {wpb, mul, nx, 0, dyn};
                     }
                 }
         best.trials = trials;
@@ -885,6 +922,7 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
     a.tune_wpb = it->second.wpb;
     a.tune_grid_mul = it->second.grid_mul;
     a.no_xcd_order = it->second.no_xcd;
+    a.tile_ctr = it->second.dyn ? (uint32_t *)(work + 7) : nullptr;
 }
 
 // Options beyond the reference's run: the learned candidate filter, and a
@@ -948,8 +986,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemsetAsync(small, 0, 128 * sizeof(uint32_t), s), "memset");
     uint64_t *key = buf<uint64_t>(c, S_KEY, N), *key2 = buf<uint64_t>(c, S_KEY2, N);
     uint32_t *idx = buf<uint32_t>(c, S_IDX, N), *idx2 = buf<uint32_t>(c, S_IDX2, N);
-    const uint64_t hmask = (p->debug_hash_bits == 0 || p->debug_hash_bits >= 64) ? ~0ull
-                                                                                 : ((1ull << p->debug_hash_bits) - 1ull);
+    // 32 key bits are plenty to group duplicates (colliding rows are told apart
+    // by the byte comparison) and halve the sort passes of a 64-bit key
+    const unsigned hbits = p->debug_hash_bits == 0 ? 32u : std::min(p->debug_hash_bits, 64u);
+    const uint64_t hmask = hbits >= 64 ? ~0ull : ((1ull << hbits) - 1ull);
     hipLaunchKernelGGL(k_rowscan, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, N, hmask,
                        key, idx, small);
     uint32_t hs[16];
@@ -983,7 +1023,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         fail(c, IDB_ERR_UNSUPPORTED, "the learned filter needs Levenshtein mode and rows of at most 32 letters");
     if (learned && idb_encoder_device(x->enc) != c->device)
         fail(c, IDB_ERR_ARG, "the encoder lives on another device");
-    sort_pairs_u64(c, key, key2, idx, idx2, N, 64);
+    sort_pairs_u64(c, key, key2, idx, idx2, N, hbits);
     uint32_t *head0 = buf<uint32_t>(c, S_HEAD, N);
     uint32_t *rep = buf<uint32_t>(c, S_REP, N);
     uint32_t *copies = buf<uint32_t>(c, S_COPIES, N);
@@ -1009,26 +1049,34 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     // --- Identify core points: buckets, tiles, count pass
     const uint32_t gj = 32 * std::max(jw, 1u);
     const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
-    uint32_t U = 0;
-    readback(c, &U, num, sizeof(uint32_t));
-    uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)U + 1);
-    uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)U + 1);
+    // Memberships are scanned over N + 1 slots (0 past the U uniques), so U
+    // and the entry count come back together in one read-back.
+    uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)N + 1);
+    uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)N + 1);
+    uint32_t *sz = small + 100;
     uint32_t single = 0;
-    uint32_t E = 0;
-    // With no V or no J genes the gate of R6 fails for every pair: no entries.
+    uint32_t E = 0, U = 0;
     if (vw > 0 && jw > 0) {
-        hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)U + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb, jw,
-                           0u, memb);
-        exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)U + 1);
-        uint64_t E64 = 0;
-        readback(c, &E64, eoff + U, sizeof(uint64_t));
+        hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)N + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb, jw,
+                           0u, N, memb);
+        exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
+        hipLaunchKernelGGL(k_size_entries, dim3(1), dim3(64), 0, s, eoff, num, sz);
+    }
+    {
+        uint32_t ue[3] = {0, 0, 0};
+        check(c, hipMemcpyAsync(ue, num, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "readback");
+        if (vw > 0 && jw > 0) check(c, hipMemcpyAsync(ue + 1, sz, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "readback");
+        check(c, hipStreamSynchronize(s), "readback sync");
+        U = ue[0];
+        uint64_t E64 = (uint64_t)ue[1] | ((uint64_t)ue[2] << 32);
+        // With no V or no J genes the gate of R6 fails for every pair: no entries.
         // Rows with many genes are copied into many buckets; past 4x the
         // uniques one bucket per repertoire with an explicit gate is cheaper.
-        if (E64 > 4ull * U + 4096 || gvj64 * std::max(n_reps, 1u) >= (1ull << 47)) {
+        if (vw > 0 && jw > 0 && (E64 > 4ull * U + 4096 || gvj64 * std::max(n_reps, 1u) >= (1ull << 47))) {
             single = 1;
-            hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)U + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb,
-                               jw, 1u, memb);
-            exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)U + 1);
+            hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)N + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb,
+                               jw, 1u, N, memb);
+            exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
             E64 = U;
         }
         E = (uint32_t)E64;
@@ -1074,10 +1122,11 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     if (E > 0) {
         uint64_t *ekey = buf<uint64_t>(c, S_EKEY, E), *ekey2 = buf<uint64_t>(c, S_EKEY2, E);
         uint32_t *erow = buf<uint32_t>(c, S_EROW, E), *erow2 = buf<uint32_t>(c, S_EROW2, E);
+        const uint32_t lb = std::max(1u, bits_for(maxlen));
         hipLaunchKernelGGL(k_emit_entries, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, eoff, vb, vw, jb, jw, len, rid,
-                           single, gj, gvj, ekey, erow);
+                           single, gj, gvj, lb, ekey, erow);
         const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
-        sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, 16 + bits_for(max_bucket));
+        sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, lb + bits_for(max_bucket));
         uint16_t *elen = buf<uint16_t>(c, S_ELEN, E);
         idb_entry *ent = buf<idb_entry>(c, S_ENT, E);
         uint8_t *bflag = buf<uint8_t>(c, S_BFLAG, E);
@@ -1095,26 +1144,29 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
-                           ham ? 1u : 0u, elen, ent, ecodes, epk, bflag, eqg);
+                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg);
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
+        // Buckets B <= E and query blocks QB <= E stay on the device: the
+        // block and tile counts are scanned over E + 1 slots, and only the
+        // tile total comes back.
         uint32_t *qbcnt = buf<uint32_t>(c, S_QBCNT, (size_t)E + 1);
         uint32_t *qboff = buf<uint32_t>(c, S_QBOFF, (size_t)E + 1);
-        // bucket count B is read on the device; scans need the length, so read it
-        readback(c, &B, num + 1, sizeof(uint32_t));
-        hipLaunchKernelGGL(k_bucket_blocks, dim3(blocks_for((uint64_t)B + 1)), dim3(256), 0, s, bstart, num + 1, E, qbcnt);
-        exclusive_scan_total<uint32_t>(c, qbcnt, qboff, (size_t)B + 1);
-        uint32_t QB = 0;
-        readback(c, &QB, qboff + B, sizeof(uint32_t));
-        uint32_t *tcount = buf<uint32_t>(c, S_TCNT, (size_t)QB + 1);
-        uint32_t *toff = buf<uint32_t>(c, S_TOFF, (size_t)QB + 1);
-        hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for((uint64_t)QB + 1)), dim3(256), 0, s, qboff, num + 1, bstart, E,
+        hipLaunchKernelGGL(k_bucket_blocks, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, bstart, num + 1, E, qbcnt);
+        exclusive_scan_total<uint32_t>(c, qbcnt, qboff, (size_t)E + 1);
+        uint32_t *tcount = buf<uint32_t>(c, S_TCNT, (size_t)E + 1);
+        uint32_t *toff = buf<uint32_t>(c, S_TOFF, (size_t)E + 1);
+        hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, qboff, num + 1, bstart, E,
                            elen, hilen, banded ? 1u : 0u, tcount);
-        exclusive_scan_total<uint32_t>(c, tcount, toff, (size_t)QB + 1);
-        readback(c, &T, toff + QB, sizeof(uint32_t));
+        exclusive_scan_total<uint32_t>(c, tcount, toff, (size_t)E + 1);
+        hipLaunchKernelGGL(k_size_tiles, dim3(1), dim3(64), 0, s, num, qboff, toff, sz);
+        uint32_t bqt[3];
+        readback(c, bqt, sz + 2, sizeof(bqt));
+        B = bqt[0];
+        T = bqt[2];
         idb_tile *tiles = buf<idb_tile>(c, S_TILES, std::max<uint32_t>(T, 1u));
-        hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(QB)), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
-                           banded ? 1u : 0u, ekey2, gvj, toff, tiles);
+        hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(bqt[1])), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
+                           banded ? 1u : 0u, ekey2, lb, gvj, toff, tiles);
         a.tiles = tiles;
         a.n_tiles = T;
         a.ent = ent;
@@ -1337,7 +1389,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->fuse_from_edges = from_edges ? 1u : 0u;
         st->tuned_wpb = a.tune_wpb;
         st->tuned_grid_mul = a.tune_grid_mul;
-        st->tuned_xcd_order = a.tune_wpb ? (a.no_xcd_order ? 0u : 1u) : 0u;
+        st->tuned_xcd_order = a.tune_wpb ? ((a.no_xcd_order ? 0u : 1u) | (a.tile_ctr ? 2u : 0u)) : 0u;
         st->tuned_trials = 0;
         if (p->flags & IDB_FLAG_AUTOTUNE)
             for (const auto &kv : c->tuned)

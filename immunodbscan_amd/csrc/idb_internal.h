@@ -88,14 +88,19 @@ struct idb_pair_args {
     // launch shape (0 = default): waves per block, grid = resident blocks x
     // grid_mul, and plain block order instead of the XCD-grouped one
     uint32_t tune_wpb, tune_grid_mul, no_xcd_order;
+    // dynamic tile order (count pass of the Myers kernels): after its first
+    // tile a wave claims the next one from this counter (zeroed per launch);
+    // null = static order
+    uint32_t *tile_ctr;
 };
 
 // LDS bytes one wave of the Myers kernels needs: Peq columns, the edge
-// staging slots, the Myers queue (2 * IDB_TILE 16-bit entries) and the
-// per-tile count accumulators of queries and targets.
+// staging slots, the Myers queue (2 * IDB_TILE 16-bit entries), the per-tile
+// count accumulators of queries and targets, and the tile's query records.
 __host__ __device__ inline size_t idb_myers_lds_per_wave(uint32_t K, uint32_t W)
 {
-    return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t);
+    return ((size_t)K * W * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t) +
+           IDB_TILE * sizeof(idb_entry);
 }
 // Extra LDS per wave for the query profiles of the 3-gram filter.
 __host__ __device__ inline size_t idb_qgram_lds_per_wave() { return (size_t)IDB_TILE * IDB_QGRAM_WORDS * sizeof(uint32_t); }

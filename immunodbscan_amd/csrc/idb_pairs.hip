@@ -198,6 +198,16 @@ __device__ __forceinline__ bool fuse_skipped(const idb_pair_args &a)
 // the imbalance it removes.
 __device__ __forceinline__ uint32_t next_tile(uint32_t ti, uint32_t total_waves) { return ti + total_waves; }
 
+// Dynamic order: wave w starts on tile w and claims each further tile from a
+// counter.  The claim is issued once the tile's own loads have been consumed,
+// so its round trip overlaps the tile's query loop.
+__device__ __forceinline__ uint32_t claim_tile(uint32_t *ctr, uint32_t lane)
+{
+    uint32_t c = 0;
+    if (lane == 0) c = atomicAdd(ctr, 1u);
+    return c;
+}
+
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
 // starts on a contiguous run of tiles (neighbouring tiles share query blocks).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
-    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, kept in a scalar register
     const uint32_t K = a.K;
     constexpr uint32_t QG_WORDS = FILT == 2 ? IDB_TILE * IDB_QGRAM_WORDS : 0u;
     uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t) + QG_WORDS);
@@ -323,12 +333,14 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
     int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
     int32_t *tacc = qacc + IDB_TILE;
+    idb_entry *qrec = (idb_entry *)(tacc + IDB_TILE);
     if (FUSE && fuse_skipped(a)) return;
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
 
-    for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
+    const bool dyn = !FUSE && a.tile_ctr;
+    for (uint32_t ti = wave0; ti < a.n_tiles;) {
         const idb_tile tile = a.tiles[ti];
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
@@ -373,6 +385,10 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             qacc[lane] = 0;
             tacc[lane] = 0;
         }
+        // query records for the loop: one LDS broadcast per query instead of
+        // a chain of lane reads into scalar registers
+        qrec[lane] = qe;
+        const uint32_t claim = dyn ? claim_tile(a.tile_ctr, lane) : 0u;
         uint32_t tq[FILT == 2 ? IDB_QGRAM_WORDS : 1];
         if (FILT == 2) {
             const uint4 *src = (const uint4 *)(a.e_qg + (size_t)p * IDB_QGRAM_WORDS);
@@ -393,12 +409,14 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         }
 
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         for (uint32_t qi = 0; qi < qn; ++qi) {
-            const uint32_t x = rl(qe.row, qi);
-            const uint32_t lc = rl(qe.len_chk, qi);
+            const idb_entry q = qrec[qi];
+            const uint32_t x = q.row;
+            const uint32_t lc = q.len_chk;
             const uint32_t lx = lc & 0xFFFFu;
             const bool chk_x = (lc >> 16) & 1u;
-            const int32_t t = !tvalid ? -1 : (lx >= m ? t_of(rl(qe.t2, qi), FUSE) : t_m);
+            const int32_t t = !tvalid ? -1 : (lx >= m ? t_of(q.t2, FUSE) : t_m);
             const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
             bool valid = tvalid && (!diag || lane > qi) && dl <= t;
             int32_t core_x = 0;
@@ -406,7 +424,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 core_x = (int32_t)rl((uint32_t)q_core, qi);
                 valid = valid && (core_x | core_y);
             }
-            if (valid) valid = gate_pair(a, chk_x, chk_y, rl(qe.sig, qi), te.sig, x, y, tile.bucket);
+            if (valid) valid = gate_pair(a, chk_x, chk_y, q.sig, te.sig, x, y, tile.bucket);
             const unsigned long long vm = __ballot(valid);
             if (!vm) continue;
             gated += __popcll(vm);
@@ -420,7 +438,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                     // position, so lev(x, y) >= #{i : x_i not in y} (and
                     // symmetrically).  The planes hold the letters seen at
                     // least once / twice.
-                    const uint32_t p1x = rl(qe.pres1, qi), p2x = rl(qe.pres2, qi);
+                    const uint32_t p1x = q.pres1, p2x = q.pres2;
                     const uint32_t lbx = __builtin_popcount(p1x & ~te.pres1) + __builtin_popcount(p2x & ~te.pres1);
                     const uint32_t lby = __builtin_popcount(te.pres1 & ~p1x) + __builtin_popcount(te.pres2 & ~p1x);
                     const uint32_t lb = lbx > lby ? lbx : lby;
@@ -486,7 +504,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (!FUSE) {
                 hits += __popcll(hm);
                 if (hit) {
-                    acc_y += (int32_t)rl(qe.cp, qi);
+                    acc_y += (int32_t)q.cp;
                     atomicAdd(qacc + qi, (int32_t)cy);
                 }
                 stage_edges(a, est, hit, x, y, lane);
@@ -509,6 +527,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
             if (acc_y) atomicAdd(a.cnt + y, acc_y);
         }
+        ti = dyn ? total_waves + __builtin_amdgcn_readfirstlane(claim) : next_tile(ti, total_waves);
     }
     if (!FUSE) finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
@@ -547,11 +566,11 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
-    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, kept in a scalar register
     const uint32_t K = a.K;
     constexpr int W = 1;
     // per wave: Peq, edge stage, queue, count accumulators, query records
-    uint32_t *peq = lds + (size_t)wib * ((idb_myers_lds_per_wave(K, W) + IDB_TILE * sizeof(idb_entry)) / sizeof(uint32_t));
+    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
     EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
     uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
     int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
@@ -760,7 +779,7 @@ template <int HWMAX, int BITS, bool FUSE>
 __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t wpb)
 {
     const uint32_t lane = lane_id();
-    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, kept in a scalar register
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
     const uint32_t HW = a.HW;
@@ -1040,7 +1059,7 @@ hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
 hipError_t launch_learned(const idb_pair_args &a, hipStream_t s)
 {
     if (a.n_tiles == 0) return hipSuccess;
-    const size_t per_wave = idb_myers_lds_per_wave(a.K, 1) + IDB_TILE * sizeof(idb_entry);
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, 1);
     uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
     if (wpb > 4) wpb = 4;
     if (wpb == 0) return hipErrorInvalidValue;

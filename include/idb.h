@@ -61,15 +61,16 @@ typedef struct {
     uint32_t min_pts;             /* minimum_number_of_neighbors                               */
     uint32_t use_hamming;         /* use_hamming                                               */
     uint32_t tree_stats;          /* get_tree_depth_statistics                                 */
-    uint32_t debug_hash_bits;     /* 0 = full 64-bit duplicate hash; 1..63 keeps only that
-                                     many bits (forces collisions in tests)                   */
+    uint32_t debug_hash_bits;     /* 0 = 32-bit duplicate hash; 1..64 keeps that many bits
+                                     (few bits force collisions in tests)                     */
     uint32_t edge_capacity;       /* 0 = automatic; else the most neighbour pairs kept from the
                                      count pass (more fall back to a second pass over pairs)  */
     uint32_t flags;               /* IDB_FLAG_* below (0: plain launches)                      */
 } idb_params;
 
 /* flags: AUTOTUNE times the count kernel's launch shapes (waves per block,
- * grid size, XCD tile order) on the first run of each kernel and shape class
+ * grid size, XCD block order, static or dynamic tile order) on the first run
+ * of each kernel and shape class
  * in a context and keeps the fastest; GRAPH captures everything after the
  * sizing steps (count pass, core flags, cluster growth, renumbering, outputs)
  * into one HIP graph per context, updated in place on later runs, with the
@@ -110,7 +111,9 @@ typedef struct {
     float fuse_kernel_ms;         /* fuse-pass kernel                                */
     uint32_t tree_depth[51];      /* union-find depth histogram when tree_stats      */
     float stage_ms[IDB_N_STAGES]; /* with IDB_FLAG_GRAPH the graph's time is the core-point stage's */
-    uint32_t tuned_wpb, tuned_grid_mul, tuned_xcd_order, tuned_trials;  /* count-kernel launch shape */
+    uint32_t tuned_wpb, tuned_grid_mul;  /* count-kernel launch shape: waves per block, grid / resident */
+    uint32_t tuned_xcd_order;     /* bit 0: XCD-grouped block order; bit 1: dynamic tile order */
+    uint32_t tuned_trials;        /* shapes the autotuner timed                      */
     uint32_t graph_used;          /* 1: the tail ran as one graph launch             */
     float encoder_ms;             /* learned filter: embedding of every row (HIP events) */
 } idb_stats;
