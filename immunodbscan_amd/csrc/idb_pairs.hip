@@ -941,14 +941,31 @@ __global__ void k_union_edges(const uint2 *edges, const uint32_t *fill, const ui
     if (nc > max_chunks) return;
     const unsigned long long n = (unsigned long long)nc << chunk_log2;
     const uint32_t cmask = (1u << chunk_log2) - 1u;
-    for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
-         e += (unsigned long long)gridDim.x * blockDim.x) {
-        if (((uint32_t)e & cmask) >= fill[e >> chunk_log2]) continue;
-        const uint2 p = edges[e];
-        const int32_t cx = core[p.x], cy = core[p.y];
-        if (cx && cy) union_cached(par, p.x, p.y);
-        else if (cx) atomicMin(bmin + p.y, p.x);
-        else if (cy) atomicMin(bmin + p.x, p.y);
+    // The grid-stride step is a multiple of 64 and chunks hold a multiple of
+    // 64 pairs, so the lanes of a wave always hold consecutive slots of one
+    // chunk.  Consecutive pairs mostly share their first row x (one query's
+    // hits); a core-core run x-y1, x-y2, ... is joined as the chain x-y1,
+    // y1-y2, ...: the same component, without every lane queueing its
+    // exchange on the root of x.
+    const uint32_t lane = __lane_id();
+    for (unsigned long long base = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) & ~63ull; base < n;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long e = base + lane;
+        const bool live = ((uint32_t)e & cmask) < fill[e >> chunk_log2];
+        uint2 p = make_uint2(IDB_NONE, IDB_NONE);
+        int32_t cx = 0, cy = 0;
+        if (live) {
+            p = edges[e];
+            cx = core[p.x];
+            cy = core[p.y];
+        }
+        const bool cc = live && cx && cy;
+        // previous lane's pair, read with the whole wave active
+        const uint32_t px = (uint32_t)__shfl_up((int)p.x, 1, 64), py = (uint32_t)__shfl_up((int)p.y, 1, 64);
+        const bool pcc = __shfl_up((int)cc, 1, 64) != 0 && lane > 0;
+        if (cc) union_cached(par, (pcc && px == p.x) ? py : p.x, p.y);
+        else if (live && cx) atomicMin(bmin + p.y, p.x);
+        else if (live && cy) atomicMin(bmin + p.x, p.y);
     }
 }
 
