@@ -413,10 +413,12 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         for (uint32_t qi = 0; qi < qn; ++qi) {
             const idb_entry q = qrec[qi];
             const uint32_t x = q.row;
-            const uint32_t lc = q.len_chk;
+            // the query's length and thresholds are wave-uniform: scalar registers
+            const uint32_t lc = __builtin_amdgcn_readfirstlane(q.len_chk);
             const uint32_t lx = lc & 0xFFFFu;
             const bool chk_x = (lc >> 16) & 1u;
-            const int32_t t = !tvalid ? -1 : (lx >= m ? t_of(q.t2, FUSE) : t_m);
+            const int32_t t_q = t_of(__builtin_amdgcn_readfirstlane(q.t2), FUSE);
+            const int32_t t = !tvalid ? -1 : (lx >= m ? t_q : t_m);
             const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
             bool valid = tvalid && (!diag || lane > qi) && dl <= t;
             int32_t core_x = 0;
