@@ -372,12 +372,21 @@ def main() -> int:
             "algorithmic_ops": ops,
         },
     }
+    # The side sections never take the headline line down with them: a
+    # failure is reported in its place.
+    def section(name, fn):
+        try:
+            out[name] = fn()
+        except Exception as exc:  # noqa: BLE001
+            log(f"{name} section failed: {exc!r}")
+            out[name] = {"error": repr(exc)}
+
     if not args.no_learned:
-        out["learned"] = learned_section(args, cl, rep, ms, run.result())
+        section("learned", lambda: learned_section(args, cl, rep, ms, run.result()))
     if not args.no_batched:
-        out["batched"] = batched_section(args)
+        section("batched", lambda: batched_section(args))
     if args.c5_rows:
-        out["large"] = large_section(args)
+        section("large", lambda: large_section(args))
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_rows, n)} rows, {threads} threads")
