@@ -49,20 +49,26 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1): one hardware exponential and one
+// reciprocal (exp overflows to inf for large u and gives 1; underflows to 0
+// for very negative u and gives -1).  Its error is far below the bfloat16
+// rounding every activation goes through.
+__device__ __forceinline__ float tanh_exp(float u) { return 1.f - __fdividef(2.f, __expf(2.f * u) + 1.f); }
+
 __device__ __forceinline__ float gelu(float x)
 {
-    return 0.5f * x * (1.f + tanhf(GELU_K * (x + 0.044715f * x * x * x)));
+    return 0.5f * x * (1.f + tanh_exp(GELU_K * (x + 0.044715f * x * x * x)));
 }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float th = tanhf(GELU_K * (x + 0.044715f * x * x * x));
+    const float th = tanh_exp(GELU_K * (x + 0.044715f * x * x * x));
     return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * GELU_K * (1.f + 3.f * 0.044715f * x * x);
 }
 
 // ---------------------------------------------------------------------------
 // GEMM: out[m][n] = sum_k A[m][k] B[n][k] over a block tile of 128 x 128 with
-// k chunks of 64, double-buffered in LDS (the next chunk's global loads are in
-// flight while the current chunk's 16 MFMAs per wave run).  Four waves own
+// k chunks of 64, double-buffered in LDS, with global loads two chunks ahead
+// of the chunk whose 16 MFMAs per wave are running.  Four waves own
 // 64 x 64 each (2 x 2 MFMA tiles of 32 x 32).  LDS rows hold 64 k values plus
 // 8 of padding (144 bytes), so the 16-byte fragment reads of 16 consecutive
 // rows hit disjoint banks.  Loaders fetch 8 consecutive k of one row (ROW) or
@@ -118,18 +124,22 @@ __global__ __launch_bounds__(256, 2) void k_gemm(LA la, LB lb, EP ep, uint32_t M
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
     if (kb >= ke) return;
-    uint4 ra[GCH], rb[GCH];
+    // Loads run two chunks ahead: chunk c + 1 sits in registers (ra, rb)
+    // while chunk c + 2 is in flight (na, nb) and chunk c is consumed from LDS.
+    uint4 ra[GCH], rb[GCH], na[GCH] = {}, nb[GCH] = {};
     fetch_tile<LA::COL>(la, m0, kb, ke, tid, ra);
     fetch_tile<LB::COL>(lb, n0, kb, ke, tid, rb);
     store_tile<LA::COL>(As[0], ra, tid);
     store_tile<LB::COL>(Bs[0], rb, tid);
+    fetch_tile<LA::COL>(la, m0, kb + GK, ke, tid, ra);
+    fetch_tile<LB::COL>(lb, n0, kb + GK, ke, tid, rb);
     __syncthreads();
     uint32_t buf = 0;
     for (uint32_t k0 = kb; k0 < ke; k0 += GK) {
         const bool more = k0 + GK < ke;
-        if (more) {
-            fetch_tile<LA::COL>(la, m0, k0 + GK, ke, tid, ra);
-            fetch_tile<LB::COL>(lb, n0, k0 + GK, ke, tid, rb);
+        if (k0 + 2 * GK < ke) {
+            fetch_tile<LA::COL>(la, m0, k0 + 2 * GK, ke, tid, na);
+            fetch_tile<LB::COL>(lb, n0, k0 + 2 * GK, ke, tid, nb);
         }
 #pragma unroll
         for (uint32_t ks = 0; ks < GK / 16; ++ks) {
@@ -149,6 +159,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm(LA la, LB lb, EP ep, uint32_t M
         if (more) {
             store_tile<LA::COL>(As[buf ^ 1], ra, tid);
             store_tile<LB::COL>(Bs[buf ^ 1], rb, tid);
+#pragma unroll
+            for (uint32_t i = 0; i < GCH; ++i) {
+                ra[i] = na[i];
+                rb[i] = nb[i];
+            }
         }
         __syncthreads();
         buf ^= 1;
@@ -158,14 +173,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm(LA la, LB lb, EP ep, uint32_t M
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t row = m0 + wm + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const uint32_t col = n0 + wn + j * 32 + (lane & 31);
-                if (row < M && col < N) ep(row, col, acc[i][j][r]);
-            }
+        for (int j = 0; j < 2; ++j) ep.tile(acc[i][j], m0 + wm + i * 32, n0 + wn + j * 32 + (lane & 31), lane, M, N);
 }
+
+// Row of element r of a lane's 32 x 32 result tile, relative to the tile.
+__device__ __forceinline__ uint32_t tile_row(int r, uint32_t lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
 // im2col window of an activation matrix X[rows][cin]: element (m, t * cin +
 // c) is X[32 s + q][c] with q = p + t - 2 (forward) or q = p - t + 2 (data
@@ -227,17 +239,27 @@ struct LdConvT {
     }
 };
 
+// Epilogues get one 32 x 32 result tile per call: rows row0 .. row0 + 31
+// (one sequence: row0 is a multiple of 32) and this lane's column `col`.
+
 // Forward epilogue: bias, saved pre-activation, GELU, length mask.
 struct EpConvFwd {
     const uint16_t *bias;
     const uint32_t *len;
     uint16_t *H, *A;
-    __device__ void operator()(uint32_t m, uint32_t o, float v) const
+    __device__ void tile(const f32x16 &v, uint32_t row0, uint32_t col, uint32_t lane, uint32_t M, uint32_t N) const
     {
-        v += bf2f(bias[o]);
-        const bool valid = (m & 31u) < len[m >> 5];
-        if (H) H[(size_t)m * C + o] = f2bf(v);
-        A[(size_t)m * C + o] = valid ? f2bf(gelu(v)) : (uint16_t)0;
+        if (col >= N || row0 >= M) return;
+        const float b = bf2f(bias[col]);
+        const uint32_t L = len[row0 >> 5];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t p = tile_row(r, lane);
+            const size_t at = (size_t)(row0 + p) * C + col;
+            const float x = v[r] + b;
+            if (H) H[at] = f2bf(x);
+            A[at] = p < L ? f2bf(gelu(x)) : (uint16_t)0;
+        }
     }
 };
 
@@ -249,18 +271,33 @@ struct EpConvBwd {
     uint32_t cin;
     uint16_t *dHprev;
     float *dX0;
-    __device__ void operator()(uint32_t m, uint32_t c, float v) const
+    __device__ void tile(const f32x16 &v, uint32_t row0, uint32_t col, uint32_t lane, uint32_t M, uint32_t N) const
     {
-        const bool valid = (m & 31u) < len[m >> 5];
-        if (dHprev) dHprev[(size_t)m * cin + c] = valid ? f2bf(v * gelu_grad(bf2f(Hprev[(size_t)m * cin + c]))) : (uint16_t)0;
-        else dX0[(size_t)m * cin + c] = valid ? v : 0.f;
+        if (col >= N || row0 >= M) return;
+        const uint32_t L = len[row0 >> 5];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t p = tile_row(r, lane);
+            const size_t at = (size_t)(row0 + p) * cin + col;
+            const bool valid = p < L;
+            if (dHprev) dHprev[at] = valid ? f2bf(v[r] * gelu_grad(bf2f(Hprev[at]))) : (uint16_t)0;
+            else dX0[at] = valid ? v[r] : 0.f;
+        }
     }
 };
 
 struct EpAtomic {
     float *G;
     uint32_t ld;
-    __device__ void operator()(uint32_t r, uint32_t c, float v) const { atomicAdd(G + (size_t)r * ld + c, v); }
+    __device__ void tile(const f32x16 &v, uint32_t row0, uint32_t col, uint32_t lane, uint32_t M, uint32_t N) const
+    {
+        if (col >= N) return;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t row = row0 + tile_row(r, lane);
+            if (row < M) atomicAdd(G + (size_t)row * ld + col, v[r]);
+        }
+    }
 };
 
 // ---------------------------------------------------------------------------
