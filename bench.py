@@ -186,7 +186,7 @@ def batched_section(args) -> dict:
         "tiles": s["tiles"],
         "graph_used": s["graph_used"],
         "tuned_launch": {"waves_per_block": s["tuned_wpb"], "grid_x_resident": s["tuned_grid_mul"],
-                         "xcd_order": s["tuned_xcd_order"] & 1, "dynamic_tiles": s["tuned_xcd_order"] >> 1, "candidates_timed": s["tuned_trials"]},
+                         "xcd_order": s["tuned_xcd_order"] & 1, "candidates_timed": s["tuned_trials"]},
         "stage_ms": s["stage_ms"],
     }
 

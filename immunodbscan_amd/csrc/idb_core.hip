@@ -57,7 +57,7 @@ struct idb_ctx {
     hipEvent_t kev[4] = {};       // around the count and the fuse kernels
     // autotuner: best count-kernel launch shape per kernel and shape class
     struct Tuned {
-        uint32_t wpb, grid_mul, no_xcd, trials, dyn;
+        uint32_t wpb, grid_mul, no_xcd, trials;
     };
     std::map<std::array<uint32_t, 6>, Tuned> tuned;
     hipGraphExec_t gexec = nullptr;  // the tail graph, updated in place
@@ -855,11 +855,10 @@ void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "
 
 // Autotuner of the count kernel's launch shape: waves per block (1, 2, 4 as
 // LDS allows), grid of 1, 2, 4 or 8 times the resident blocks, XCD-grouped or
-// plain block order, and (Myers kernels) static or dynamic tile order.  Each
-// candidate runs the real count pass on the real tiles twice (counts restored
-// from a snapshot in between) and the fastest by HIP events is kept for this
-// kernel and shape class (kernel, words, code width, filter, alphabet, log2
-// tiles) for the context's lifetime.
+// plain block order.  Each candidate runs the real count pass on the real
+// tiles twice (counts restored from a snapshot in between) and the fastest by
+// HIP events is kept for this kernel and shape class (kernel, words, code
+// width, filter, alphabet, log2 tiles) for the context's lifetime.
 void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bool learned, uint32_t *small,
               unsigned long long *work)
 {
@@ -871,22 +870,18 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
         hipStream_t s = c->stream;
         int32_t *snap = buf<int32_t>(c, S_CNT2, N);
         check(c, hipMemcpyAsync(snap, a.cnt, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune snapshot");
-        idb_ctx::Tuned best{0, 0, 0, 0, 0};
+        idb_ctx::Tuned best{0, 0, 0, 0};
         float best_ms = 1e30f;
         uint32_t trials = 0;
-        // dynamic tile order only for the Myers count kernel
-        const uint32_t dyn_max = (!learned && !ham) ? 1u : 0u;
-        for (uint32_t dyn = 0; dyn <= dyn_max; ++dyn)
         for (uint32_t wpb : {1u, 2u, 4u})
             for (uint32_t mul : {1u, 2u, 4u, 8u})
                 for (uint32_t nx : {0u, 1u}) {
                     a.tune_wpb = wpb;
                     a.tune_grid_mul = mul;
                     a.no_xcd_order = nx;
-                    a.tile_ctr = dyn ? (uint32_t *)(work + 7) : nullptr;
                     check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
                     check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-                    check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+                    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
                     // best of two runs per shape
                     float ms = 1e30f;
                     for (int rep = 0; rep < 2; ++rep) {
@@ -894,7 +889,7 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
                             check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s),
                                   "tune reset");
                             check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-                            check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+                            check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
                         }
                         check(c, hipEventRecord(c->kev[0], s), "event record");
                         const hipError_t e = learned ? idb_launch_count_learned(a, s)
@@ -910,19 +905,18 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
                     ++trials;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = {wpb, mul, nx, 0, dyn};
+                        best = {wpb, mul, nx, 0};
                     }
                 }
         best.trials = trials;
         check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
         check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-        check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "tune reset");
+        check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
         it = c->tuned.emplace(key, best).first;
     }
     a.tune_wpb = it->second.wpb;
     a.tune_grid_mul = it->second.grid_mul;
     a.no_xcd_order = it->second.no_xcd;
-    a.tile_ctr = it->second.dyn ? (uint32_t *)(work + 7) : nullptr;
 }
 
 // Options beyond the reference's run: the learned candidate filter, and a
@@ -1114,8 +1108,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(hilen, hl.data(), sizeof(uint16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
     check(c, hipMemcpyAsync(cmap, cmap_h, 256, hipMemcpyHostToDevice, s), "cmap");
 
-    unsigned long long *work = buf<unsigned long long>(c, S_WORK, 8);
-    check(c, hipMemsetAsync(work, 0, 8 * sizeof(unsigned long long), s), "memset");
+    unsigned long long *work = buf<unsigned long long>(c, S_WORK, IDB_WORK_U64);
+    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "memset");
     unsigned long long edge_cap = 0;
     uint32_t B = 0, T = 0;
     idb_pair_args a{};
@@ -1133,8 +1127,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         uint32_t *ecodes = ham ? nullptr : buf<uint32_t>(c, S_ECODES, (size_t)E * (S / 4));
         uint32_t *epk = ham ? buf<uint32_t>(c, S_EPK, (size_t)E * HW) : nullptr;
         // Levenshtein codes are padded with a sentinel letter K whose Peq row
-        // stays zero (with all 256 byte values present code 0 is used; no
-        // kernel reads letters past a row's end)
+        // stays zero, so a Peq read for a letter past a row's end stays in
+        // range (with all 256 byte values present code 0 is used; only the
+        // one-word count kernel reads letters past a row's end, and it runs
+        // for alphabets of at most 31 letters)
         const uint32_t pad = (!ham && K < 256) ? K : 0u;
         // The absent-letter filter pays off where random letter matches are
         // rare (protein alphabets); with 4 letters nearly every pair passes.
@@ -1389,7 +1385,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         st->fuse_from_edges = from_edges ? 1u : 0u;
         st->tuned_wpb = a.tune_wpb;
         st->tuned_grid_mul = a.tune_grid_mul;
-        st->tuned_xcd_order = a.tune_wpb ? ((a.no_xcd_order ? 0u : 1u) | (a.tile_ctr ? 2u : 0u)) : 0u;
+        st->tuned_xcd_order = a.tune_wpb ? (a.no_xcd_order ? 0u : 1u) : 0u;
         st->tuned_trials = 0;
         if (p->flags & IDB_FLAG_AUTOTUNE)
             for (const auto &kv : c->tuned)

@@ -28,6 +28,8 @@
 
 #define IDB_NONE 0xFFFFFFFFu
 #define IDB_TILE 64               // entries per query / target block (one wave)
+// The run's device counters: 8 64-bit statistics
+#define IDB_WORK_U64 8u
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
 #define IDB_MAX_HW 64             // Hamming packed words per row
 #define IDB_MAX_LEN 65535u
@@ -88,10 +90,6 @@ struct idb_pair_args {
     // launch shape (0 = default): waves per block, grid = resident blocks x
     // grid_mul, and plain block order instead of the XCD-grouped one
     uint32_t tune_wpb, tune_grid_mul, no_xcd_order;
-    // dynamic tile order (count pass of the Myers kernels): after its first
-    // tile a wave claims the next one from this counter (zeroed per launch);
-    // null = static order
-    uint32_t *tile_ctr;
 };
 
 // LDS bytes one wave of the Myers kernels needs: Peq columns, the edge

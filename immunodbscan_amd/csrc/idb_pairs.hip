@@ -194,18 +194,15 @@ __device__ __forceinline__ bool fuse_skipped(const idb_pair_args &a)
 // The grid holds only as many waves as can be resident at once (no partly
 // empty second round of waves trails the launch); wave w takes tiles w,
 // w + waves, w + 2 waves, ...  A shared tile counter balances better in
-// principle but costs a device-scope atomic round trip per tile, more than
-// the imbalance it removes.
+// principle, but its device-scope atomics wait behind the pass's other
+// atomics: measured on the 200k-row workload, claiming tiles (one counter per
+// XCD, half of the tiles static) made the count pass slower, not faster.
 __device__ __forceinline__ uint32_t next_tile(uint32_t ti, uint32_t total_waves) { return ti + total_waves; }
 
-// Dynamic order: wave w starts on tile w and claims each further tile from a
-// counter.  The claim is issued once the tile's own loads have been consumed,
-// so its round trip overlaps the tile's query loop.
-__device__ __forceinline__ uint32_t claim_tile(uint32_t *ctr, uint32_t lane)
+// The same, n_tiles past the end (no overflow near 2^32 tiles).
+__device__ __forceinline__ uint32_t step_tile(uint32_t ti, uint32_t total_waves, uint32_t n_tiles)
 {
-    uint32_t c = 0;
-    if (lane == 0) c = atomicAdd(ctr, 1u);
-    return c;
+    return ti < n_tiles && n_tiles - ti > total_waves ? ti + total_waves : n_tiles;
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
@@ -235,6 +232,41 @@ __device__ __forceinline__ idb_entry load_entry(const idb_entry *e)
     r.pres2 = b.z;
     r.pad = 0;
     return r;
+}
+
+// What a Myers tile needs from global memory, loaded one tile ahead: the
+// record, this lane's query and target entries and, for one-word rows
+// (CODES), both rows' codes (16 or 32 bytes).
+template <bool CODES>
+struct TilePre {
+    idb_tile tile;
+    idb_entry te, qe;
+    uint4 qc0, qc1, tc0, tc1;
+};
+
+template <bool CODES>
+__device__ __forceinline__ void tile_load(const idb_pair_args &a, const idb_tile &t, uint32_t lane, TilePre<CODES> &d)
+{
+    const uint32_t qn = t.shape & 0xFFu, tn = (t.shape >> 8) & 0xFFu;
+    d.tile = t;
+    d.te = idb_entry{};
+    d.qe = idb_entry{};
+    d.qc0 = d.qc1 = d.tc0 = d.tc1 = make_uint4(0u, 0u, 0u, 0u);
+    if (lane < tn) d.te = load_entry(a.ent + t.t0 + lane);
+    if (lane < qn) d.qe = load_entry(a.ent + t.q0 + lane);
+    if (CODES) {
+        const bool wide = a.S > 16;
+        if (lane < qn) {
+            const uint4 *q = (const uint4 *)(a.e_codes + (size_t)(t.q0 + lane) * a.S);
+            d.qc0 = q[0];
+            if (wide) d.qc1 = q[1];
+        }
+        if (lane < tn) {
+            const uint4 *c = (const uint4 *)(a.e_codes + (size_t)(t.t0 + lane) * a.S);
+            d.tc0 = c[0];
+            if (wide) d.tc1 = c[1];
+        }
+    }
 }
 
 // Hit bookkeeping of the batched Myers path (one pair per lane).
@@ -308,6 +340,71 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
 }
 
+// The same for rows of at most 32 letters (one word, code rows of 16 or 32
+// bytes): each lane holds its query row's codes in registers from the start
+// of the tile (qc0, qc1), so a batch takes them from lane qi with cross-lane
+// reads instead of waiting on global loads, and the unrolled letter loop
+// leaves the compiler free to issue the Peq reads of a word together.
+template <bool FUSE>
+__device__ __forceinline__ void myers_batch1(const idb_pair_args &a, const uint32_t *peq, const uint16_t *queue,
+                                             uint32_t n, uint32_t lane, const idb_entry &qe, const idb_entry &te,
+                                             int32_t core_y, int32_t q_core, EdgeStage &est, int32_t *qacc,
+                                             int32_t *tacc, unsigned long long &hits, unsigned long long &mlet,
+                                             const uint4 &qc0, const uint4 &qc1, bool wide)
+{
+    const bool act = lane < n;
+    const uint32_t e = queue[lane < n ? lane : 0u];
+    const int qi = (int)(e >> 8), tl = (int)(e & 0xFFu);
+    const uint32_t ty = (uint32_t)__shfl((int)te.row, tl, 64);
+    const uint32_t tm = (uint32_t)__shfl((int)te.len_chk, tl, 64) & 0xFFFFu;
+    const uint32_t tcy = (uint32_t)__shfl((int)te.cp, tl, 64);
+    const uint32_t tt2 = (uint32_t)__shfl((int)te.t2, tl, 64);
+    const int32_t tcore = __shfl(core_y, tl, 64);
+    const uint32_t x = (uint32_t)__shfl((int)qe.row, qi, 64);
+    const uint32_t lxq = (uint32_t)__shfl((int)qe.len_chk, qi, 64) & 0xFFFFu;
+    const uint32_t cx = (uint32_t)__shfl((int)qe.cp, qi, 64);
+    const uint32_t qt2 = (uint32_t)__shfl((int)qe.t2, qi, 64);
+    const int32_t core_x = __shfl(q_core, qi, 64);
+    uint32_t w[8];
+    w[0] = (uint32_t)__shfl((int)qc0.x, qi, 64);
+    w[1] = (uint32_t)__shfl((int)qc0.y, qi, 64);
+    w[2] = (uint32_t)__shfl((int)qc0.z, qi, 64);
+    w[3] = (uint32_t)__shfl((int)qc0.w, qi, 64);
+    if (wide) {
+        w[4] = (uint32_t)__shfl((int)qc1.x, qi, 64);
+        w[5] = (uint32_t)__shfl((int)qc1.y, qi, 64);
+        w[6] = (uint32_t)__shfl((int)qc1.z, qi, 64);
+        w[7] = (uint32_t)__shfl((int)qc1.w, qi, 64);
+    } else {
+        w[4] = w[5] = w[6] = w[7] = 0u;
+    }
+    const uint32_t lx = act ? lxq : 0u;
+    const int32_t t = !act ? -1 : (lx >= tm ? t_of(qt2, FUSE) : t_of(tt2, FUSE));
+    mlet += lx;
+    idb_myers<1> st;
+    idb_myers_init<1>(st, tm);
+    const uint32_t top_w = tm ? 0u : 0xFFFFFFFFu, top_b = tm ? (tm - 1) & 31 : 0u;
+    const uint32_t *pcol = peq + tl;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!__ballot(4u * k < lx)) break;
+        const uint32_t word = w[k];
+        uint32_t eq[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) eq[b] = pcol[((word >> (8 * b)) & 0xFFu) * 64];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            idb_myers<1> nx = st;
+            idb_myers_step<1>(nx, eq + b, top_w, top_b);
+            if (4u * k + b < lx) st = nx;
+        }
+    }
+    const int32_t dist = tm ? st.score : (int32_t)lx;
+    const bool hit = act && dist <= t;
+    if (!FUSE) hits += __popcll(__ballot(hit));
+    record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
+}
+
 // FILT selects an exact lower bound that a pair must pass before Myers;
 // surviving pairs are queued in LDS and verified by Myers 64 at a time.
 //   1 (alphabets of 8-31 letters): the absent-letter bound, a few bit
@@ -320,7 +417,7 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
 //     stays a lower bound); the L1 distance is 16 v_sad_u8 per pair, query
 //     profiles are read from LDS by broadcast.
 template <int W, bool FUSE, int FILT>
-__global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
@@ -336,20 +433,32 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
     idb_entry *qrec = (idb_entry *)(tacc + IDB_TILE);
     if (FUSE && fuse_skipped(a)) return;
     const uint32_t total_waves = gridDim.x * wpb;
-    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
+    // consecutive tiles (often of one query block, with similar work) go to
+    // different blocks and so to different CUs, not to the waves of one block
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + wib * gridDim.x;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
 
-    const bool dyn = !FUSE && a.tile_ctr;
-    for (uint32_t ti = wave0; ti < a.n_tiles;) {
-        const idb_tile tile = a.tiles[ti];
+    // static order; the next tile's data and the record of the one after are
+    // loaded while the current tile runs
+    const uint32_t T = a.n_tiles;
+    uint32_t ti = wave0;
+    uint32_t ti1 = step_tile(ti, total_waves, T);
+    idb_tile rec1 = ti1 < T ? a.tiles[ti1] : idb_tile{};
+    TilePre<FILT == 1> cur{};
+    if (ti < T) tile_load<FILT == 1>(a, a.tiles[ti], lane, cur);
+    const bool wide = a.S > 16;
+    while (ti < T) {
+        TilePre<FILT == 1> nxt{};
+        if (ti1 < T) tile_load<FILT == 1>(a, rec1, lane, nxt);
+        const uint32_t ti2 = step_tile(ti1, total_waves, T);
+        const idb_tile rec2 = ti2 < T ? a.tiles[ti2] : idb_tile{};
+        const idb_tile tile = cur.tile;
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
         const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
-        // both records of this lane in one round trip
-        idb_entry te{}, qe{};
-        if (tvalid) te = load_entry(a.ent + p);
-        if (lane < qn) qe = load_entry(a.ent + tile.q0 + lane);
+        const idb_entry te = cur.te, qe = cur.qe;
+        const uint4 qc0 = cur.qc0, qc1 = cur.qc1;
         const uint32_t y = te.row, m = te.len_chk & 0xFFFFu, cy = te.cp;
         const bool chk_y = (te.len_chk >> 16) & 1u;
         const int32_t t_m = tvalid ? t_of(te.t2, FUSE) : -1;
@@ -358,12 +467,21 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (tvalid) core_y = a.core[y];
             if (lane < qn) q_core = a.core[qe.row];
         }
-        // Peq for this lane's pattern, in its own LDS column: code words come
-        // 16 bytes at a time (S is a multiple of 16)
+        // Peq for this lane's pattern, in its own LDS column (LDS atomic ORs:
+        // no read-back between letters)
         for (uint32_t c = 0; c < K; ++c)
 #pragma unroll
             for (int w = 0; w < W; ++w) peq[(c * W + w) * 64 + lane] = 0u;
-        if (tvalid) {
+        if (FILT == 1) {
+            if (tvalid) {
+                const uint32_t ws[8] = {cur.tc0.x, cur.tc0.y, cur.tc0.z, cur.tc0.w,
+                                        cur.tc1.x, cur.tc1.y, cur.tc1.z, cur.tc1.w};
+#pragma unroll
+                for (uint32_t i = 0; i < 32; ++i)
+                    if (i < m) atomicOr(peq + ((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) * 64 + lane, 1u << i);
+            }
+        } else if (tvalid) {
+            // code words 16 bytes at a time (S is a multiple of 16)
             const uint4 *codes = (const uint4 *)(a.e_codes + (size_t)p * a.S);
             for (uint32_t i0 = 0; i0 < m; i0 += 16) {
                 const uint4 v = codes[i0 >> 4];
@@ -373,7 +491,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                     const uint32_t i = i0 + k;
                     if (i < m) {
                         const uint32_t c = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-                        peq[(c * W + (i >> 5)) * 64 + lane] |= 1u << (i & 31);
+                        atomicOr(peq + (c * W + (i >> 5)) * 64 + lane, 1u << (i & 31));
                     }
                 }
             }
@@ -381,6 +499,12 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         const uint32_t top_w = m ? (m - 1) >> 5 : 0xFFFFFFFFu, top_b = m ? (m - 1) & 31 : 0u;
         int32_t acc_y = 0;
         uint32_t qlen = 0;  // queued pairs (FILT)
+        // query qi pairs with this lane iff qi < dlim (upper triangle on the
+        // diagonal tile, nothing past the tile's targets)
+        const uint32_t dlim = !tvalid ? 0u : (diag ? lane : 64u);
+        const uint32_t ny1 = ~te.pres1;
+        // per-tile counters fit 32 bits: at most 4096 pairs of 16-bit lengths
+        uint32_t tg = 0, tgl = 0, tm = 0;
         if (!FUSE) {
             qacc[lane] = 0;
             tacc[lane] = 0;
@@ -388,7 +512,6 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         // query records for the loop: one LDS broadcast per query instead of
         // a chain of lane reads into scalar registers
         qrec[lane] = qe;
-        const uint32_t claim = dyn ? claim_tile(a.tile_ctr, lane) : 0u;
         uint32_t tq[FILT == 2 ? IDB_QGRAM_WORDS : 1];
         if (FILT == 2) {
             const uint4 *src = (const uint4 *)(a.e_qg + (size_t)p * IDB_QGRAM_WORDS);
@@ -418,19 +541,33 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             const uint32_t lx = lc & 0xFFFFu;
             const bool chk_x = (lc >> 16) & 1u;
             const int32_t t_q = t_of(__builtin_amdgcn_readfirstlane(q.t2), FUSE);
-            const int32_t t = !tvalid ? -1 : (lx >= m ? t_q : t_m);
-            const int32_t dl = lx > m ? (int32_t)(lx - m) : (int32_t)(m - lx);
-            bool valid = tvalid && (!diag || lane > qi) && dl <= t;
+            // the pair's threshold is the longer row's; |lx - m| in one v_sad_u16
+            const int32_t t = lx >= m ? t_q : t_m;
+            const int32_t dl = (int32_t)__builtin_amdgcn_sad_u16(lx, m, 0u);
+            bool valid = qi < dlim && dl <= t;
             int32_t core_x = 0;
             if (FUSE) {
                 core_x = (int32_t)rl((uint32_t)q_core, qi);
                 valid = valid && (core_x | core_y);
             }
-            if (valid) valid = gate_pair(a, chk_x, chk_y, q.sig, te.sig, x, y, tile.bucket);
+            // V/J gate: only a query that may share a lower gene (or every
+            // query in single-bucket mode) needs it, a wave-uniform test
+            if (a.single_bucket || chk_x) {
+                const uint32_t c = __builtin_amdgcn_readfirstlane(q.sig) & te.sig;
+                bool need;
+                if (a.single_bucket) {
+                    valid = valid && (c & 0xFFFFu) != 0u && (c >> 16) != 0u;
+                    need = valid;
+                } else {
+                    need = valid && chk_y && c != 0u;
+                }
+                if (__ballot(need) && need) valid = pair_in_bucket(a, x, y, tile.bucket);
+            }
             const unsigned long long vm = __ballot(valid);
             if (!vm) continue;
-            gated += __popcll(vm);
-            gletters += (unsigned long long)__popcll(vm) * lx;
+            const uint32_t nv = (uint32_t)__popcll(vm);
+            tg += nv;
+            tgl += nv * lx;
 
             if (FILT) {
                 bool pass;
@@ -440,11 +577,11 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                     // position, so lev(x, y) >= #{i : x_i not in y} (and
                     // symmetrically).  The planes hold the letters seen at
                     // least once / twice.
-                    const uint32_t p1x = q.pres1, p2x = q.pres2;
-                    const uint32_t lbx = __builtin_popcount(p1x & ~te.pres1) + __builtin_popcount(p2x & ~te.pres1);
+                    const uint32_t p1x = __builtin_amdgcn_readfirstlane(q.pres1);
+                    const uint32_t p2x = __builtin_amdgcn_readfirstlane(q.pres2);
+                    const uint32_t lbx = __builtin_popcount(p1x & ny1) + __builtin_popcount(p2x & ny1);
                     const uint32_t lby = __builtin_popcount(te.pres1 & ~p1x) + __builtin_popcount(te.pres2 & ~p1x);
-                    const uint32_t lb = lbx > lby ? lbx : lby;
-                    pass = valid && (int32_t)lb <= t;
+                    pass = valid && (int32_t)lbx <= t && (int32_t)lby <= t;
                 } else {
                     // half the profile first: its L1 distance is already a
                     // lower bound, and most pairs fail it
@@ -467,11 +604,15 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
                 if (pass) queue[qlen + below] = (uint16_t)((qi << 8) | lane);
                 qlen += (uint32_t)__popcll(pm);
-                mpairs += __popcll(pm);
+                tm += (uint32_t)__popcll(pm);
                 if (qlen >= IDB_TILE) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, qacc,
-                                         tacc, hits, mletters);
+                    if (FILT == 1)
+                        myers_batch1<FUSE>(a, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, qacc, tacc,
+                                           hits, mletters, qc0, qc1, wide);
+                    else
+                        myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est,
+                                             qacc, tacc, hits, mletters);
                     const uint32_t rest = qlen - IDB_TILE;
                     const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -481,7 +622,7 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
                 continue;
             }
 
-            mpairs += __popcll(vm);
+            tm += nv;
             mletters += valid ? lx : 0u;
             idb_myers<W> st;
             idb_myers_init<W>(st, m);
@@ -518,9 +659,16 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
         }
         if (FILT && qlen) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
-                                 mletters);
+            if (FILT == 1)
+                myers_batch1<FUSE>(a, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
+                                   mletters, qc0, qc1, wide);
+            else
+                myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc,
+                                     hits, mletters);
         }
+        gated += tg;
+        gletters += tgl;
+        mpairs += tm;
         if (!FUSE) {
             // one atomic per row and tile
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -529,7 +677,10 @@ __global__ __launch_bounds__(256) void k_pairs_myers(idb_pair_args a, uint32_t w
             if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
             if (acc_y) atomicAdd(a.cnt + y, acc_y);
         }
-        ti = dyn ? total_waves + __builtin_amdgcn_readfirstlane(claim) : next_tile(ti, total_waves);
+        cur = nxt;
+        ti = ti1;
+        ti1 = ti2;
+        rec1 = rec2;
     }
     if (!FUSE) finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
@@ -1041,7 +1192,10 @@ hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
     wpb = pick_wpb(a, wpb);
     const dim3 block(64 * wpb);
     const size_t shm = per_wave * wpb;
-    if (W == 1 && a.band_filter) return launch_myers_w<1, FUSE, 1>(a, block, shm, wpb, s);
+    if (W == 1 && a.band_filter) {
+        if (a.S > 32 || a.K > 32) return hipErrorInvalidValue;  // one-word rows, alphabets of at most 31 letters
+        return launch_myers_w<1, FUSE, 1>(a, block, shm, wpb, s);
+    }
     if (a.qgram_filter) return launch_myers_f<FUSE, 2>(a, W, block, shm, wpb, s);
     return launch_myers_f<FUSE, 0>(a, W, block, shm, wpb, s);
 }
@@ -1141,6 +1295,7 @@ size_t idb_lds_limit()
 
 hipError_t idb_launch_count_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<false>(a, W, s); }
 hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t s) { return launch_myers<true>(a, W, s); }
+
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<false>(a, s); }
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<true>(a, s); }
 

@@ -69,8 +69,7 @@ typedef struct {
 } idb_params;
 
 /* flags: AUTOTUNE times the count kernel's launch shapes (waves per block,
- * grid size, XCD block order, static or dynamic tile order) on the first run
- * of each kernel and shape class
+ * grid size, XCD block order) on the first run of each kernel and shape class
  * in a context and keeps the fastest; GRAPH captures everything after the
  * sizing steps (count pass, core flags, cluster growth, renumbering, outputs)
  * into one HIP graph per context, updated in place on later runs, with the
@@ -112,7 +111,7 @@ typedef struct {
     uint32_t tree_depth[51];      /* union-find depth histogram when tree_stats      */
     float stage_ms[IDB_N_STAGES]; /* with IDB_FLAG_GRAPH the graph's time is the core-point stage's */
     uint32_t tuned_wpb, tuned_grid_mul;  /* count-kernel launch shape: waves per block, grid / resident */
-    uint32_t tuned_xcd_order;     /* bit 0: XCD-grouped block order; bit 1: dynamic tile order */
+    uint32_t tuned_xcd_order;     /* bit 0: XCD-grouped block order */
     uint32_t tuned_trials;        /* shapes the autotuner timed                      */
     uint32_t graph_used;          /* 1: the tail ran as one graph launch             */
     float encoder_ms;             /* learned filter: embedding of every row (HIP events) */

@@ -71,3 +71,21 @@ def test_c3_batch_graph_autotune_equals_separate(gpu_ctx):
         for k in FIELDS:
             assert np.array_equal(getattr(got, k)[s:e], exp[k]), (r, k)
         assert got.n_uniques[r] == exp["n_uniques"]
+
+
+@pytest.mark.gpu
+def test_small_pair_capacity_same_results_and_statistics(gpu_ctx):
+    """A neighbour-pair capacity far below the pairs found sends cluster growth
+    to the fuse pass over the tiles: same results and the same count-pass
+    statistics as with the pair list."""
+    rep = synth.generate(configs.c2_params(n_rows=20_000, n_seeds=2_000))
+    exp = oracle_of(rep, configs.C2, fast=True)
+    big = engine.dbscan(rep, configs.C2)
+    small = engine.dbscan(rep, configs.C2, edge_capacity=2048)
+    assert big.stats["neighbour_pairs"] > 2048 and big.stats["fuse_from_edges"] == 1
+    assert small.stats["fuse_from_edges"] == 0
+    for k in FIELDS:
+        assert np.array_equal(getattr(big, k), exp[k]), k
+        assert np.array_equal(getattr(small, k), exp[k]), k
+    for k in ("gated_pairs", "neighbour_pairs", "myers_pairs", "myers_letters", "gated_letters"):
+        assert big.stats[k] == small.stats[k], k
