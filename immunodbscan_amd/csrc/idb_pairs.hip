@@ -533,7 +533,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        for (uint32_t qi = 0; qi < qn; ++qi) {
+        if (FILT == 1 && !FUSE) {
+            // Count pass over one-word rows: the query's fields come from
+            // lane qi by v_readlane (no LDS round trip), and every test runs
+            // for every lane and becomes a mask, so a query costs one branch
+            // unless it may share a lower gene or has a pair for Myers.
+            const uint32_t y1 = te.pres1, y2 = te.pres2;
+            for (uint32_t qi = 0; qi < qn; ++qi) {
+                const uint32_t lc = __builtin_amdgcn_readlane(qe.len_chk, qi);
+                const uint32_t lx = lc & 0xFFFFu;
+                const int32_t t_q = t_of(__builtin_amdgcn_readlane(qe.t2, qi), false);
+                const uint32_t p1x = __builtin_amdgcn_readlane(qe.pres1, qi);
+                const uint32_t p2x = __builtin_amdgcn_readlane(qe.pres2, qi);
+                const int32_t t = lx >= m ? t_q : t_m;
+                const int32_t dl = (int32_t)__builtin_amdgcn_sad_u16(lx, m, 0u);
+                // absent-letter bound, both directions (see below)
+                const uint32_t lbx = __builtin_popcount(p1x & ny1) + __builtin_popcount(p2x & ny1);
+                const uint32_t lby = __builtin_popcount(y1 & ~p1x) + __builtin_popcount(y2 & ~p1x);
+                const int32_t lb = (int32_t)(lbx > lby ? lbx : lby);
+                unsigned long long vm = __ballot(qi < dlim && dl <= t);
+                if (vm && (a.single_bucket || ((lc >> 16) & 1u))) {
+                    const uint32_t c = __builtin_amdgcn_readlane(qe.sig, qi) & te.sig;
+                    const bool valid = (vm >> lane) & 1ull;
+                    bool ok = true, need;
+                    if (a.single_bucket) {
+                        ok = (c & 0xFFFFu) != 0u && (c >> 16) != 0u;
+                        need = valid && ok;
+                    } else {
+                        need = valid && chk_y && c != 0u;
+                    }
+                    const uint32_t x = __builtin_amdgcn_readlane(qe.row, qi);
+                    if (__ballot(need) && need) ok = pair_in_bucket(a, x, y, tile.bucket);
+                    vm &= __ballot(ok);
+                }
+                const uint32_t nv = (uint32_t)__popcll(vm);
+                tg += nv;
+                tgl += nv * lx;
+                const unsigned long long pm = vm & __ballot(lb <= t);
+                if (!pm) continue;
+                if ((pm >> lane) & 1ull) {
+                    const uint32_t below =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                    queue[qlen + below] = (uint16_t)((qi << 8) | lane);
+                }
+                qlen += (uint32_t)__popcll(pm);
+                tm += (uint32_t)__popcll(pm);
+                if (qlen >= IDB_TILE) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    myers_batch1<FUSE>(a, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
+                                       mletters, qc0, qc1, wide);
+                    const uint32_t rest = qlen - IDB_TILE;
+                    const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    if (lane < rest) queue[lane] = moved;
+                    qlen = rest;
+                }
+            }
+        }
+        for (uint32_t qi = (FILT == 1 && !FUSE) ? qn : 0u; qi < qn; ++qi) {
             const idb_entry q = qrec[qi];
             const uint32_t x = q.row;
             // the query's length and thresholds are wave-uniform: scalar registers
