@@ -191,6 +191,74 @@ def batched_section(args) -> dict:
     }
 
 
+def genes_section(args) -> dict:
+    """R2 on the device: IMGT-style V calls (1-3 alleles of the 104 reference
+    V gene names each) against those names, arrays resident in HBM; the host
+    builder on a 200k-row sample beside it."""
+    import random
+
+    import numpy as np
+
+    from immunodbscan_amd import _lib, engine, hip
+    from immunodbscan_amd.interlink import gene_bitsets, pack_strings
+
+    names = [ln for ln in open(os.path.join(ROOT, "tests", "golden", "VHgenes.txt"), "rb").read().split(b"\n") if ln]
+    rng = random.Random(21)
+    pool = []
+    for _ in range(4096):
+        k = rng.randrange(1, 4)
+        pool.append(b", or ".join(b"Homsap " + rng.choice(names)[:-1] + b"*0%d F" % rng.randrange(1, 9)
+                                  for _ in range(k)))
+    pd, po, pl = pack_strings(pool)
+    n = args.gene_rows
+    pick = np.random.default_rng(22).integers(0, len(pool), n)
+    lens = pl[pick].astype(np.uint32)
+    off = np.zeros(n, np.uint64)
+    off[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    data = np.empty(int(lens.sum(dtype=np.uint64)), np.uint8)
+    for a in range(0, n, 1 << 20):  # materialise every row's bytes, a million rows at a time
+        b = min(n, a + (1 << 20))
+        rl = lens[a:b].astype(np.int64)
+        base = int(off[a])
+        src = np.repeat(po[pick[a:b]].astype(np.int64) - (off[a:b].astype(np.int64) - base), rl)
+        data[base:base + int(rl.sum())] = pd[src + np.arange(int(rl.sum()))]
+    width = (len(names) + 31) // 32
+    gd, go, gl = pack_strings(names)
+    ctx = engine.default_context()
+    dd, do, dl = (hip.DeviceBuffer.from_array(x) for x in (data, off, lens))
+    dout = hip.DeviceBuffer.zeros(4 * n * width)
+    call = lambda: ctx.check(_lib.lib.idb_gene_bitsets_gpu(  # noqa: E731
+        ctx.handle, _lib.IDB_MEM_DEVICE, n, dd.data_ptr(), do.data_ptr(), dl.data_ptr(), len(names), gd.ctypes.data,
+        go.ctypes.data, gl.ctypes.data, dout.data_ptr(), width))
+    call()
+    hip.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        call()
+        hip.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ms = 1e3 * float(np.median(ts))
+    # spot check against the host builder on the first rows
+    got = dout.to_numpy(np.int32, 20_000 * width).reshape(20_000, width)
+    first = [bytes(data[int(off[i]):int(off[i]) + int(lens[i])]) for i in range(200_000)]
+    t0 = time.perf_counter()
+    host, _ = gene_bitsets(first, names)
+    host_s = time.perf_counter() - t0
+    if not np.array_equal(got, host[:20_000]):
+        raise RuntimeError("device gene bitsets differ from the host builder")
+    log(f"genes: {n} calls x {len(names)} names in {ms:.2f} ms")
+    return {
+        "workload": f"{n} IMGT-style V calls (1-3 alleles each, {data.size / n:.1f} bytes mean) x {len(names)} reference V gene names",
+        "rows": n,
+        "ms_per_call": ms,
+        "calls_per_sec": n / (ms * 1e-3),
+        "bytes_per_sec": data.size / (ms * 1e-3),
+        "cpu_baseline": {"calls_per_sec": 200_000 / host_s, "cores": 1, "kind": "port",
+                         "sample": "first 200,000 calls, host builder (std::string_view::find per name)"},
+    }
+
+
 def pmc_traffic(rows: int):
     """HBM bytes per launch of the count kernel from the committed PMC
     profile of this workload (FETCH_SIZE and WRITE_SIZE passes, each scaled
@@ -273,6 +341,7 @@ def main() -> int:
     ap.add_argument("--c4-rows", type=int, default=2_000_000, help="0 skips the 2M-row learned comparison")
     ap.add_argument("--c3-reps", type=int, default=512)
     ap.add_argument("--c5-rows", type=int, default=20_000_000, help="0 skips the 20M-row nucleotide line")
+    ap.add_argument("--gene-rows", type=int, default=10_000_000, help="0 skips the device gene-bitset line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB "
                          "(the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -387,6 +456,8 @@ def main() -> int:
         section("batched", lambda: batched_section(args))
     if args.c5_rows:
         section("large", lambda: large_section(args))
+    if args.gene_rows:
+        section("genes", lambda: genes_section(args))
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_rows, n)} rows, {threads} threads")

@@ -41,6 +41,7 @@ EXPORTED = [
     "idb_destroy",
     "idb_last_error",
     "idb_gene_bitsets",
+    "idb_gene_bitsets_gpu",
     "idb_dbscan",
     "idb_dbscan_batch",
     "idb_pair_distances",
@@ -177,6 +178,8 @@ def _load() -> C.CDLL:
     lib.idb_last_error.restype = C.c_char_p
     lib.idb_gene_bitsets.argtypes = [u32, vp, vp, vp, u32, vp, vp, vp, vp, u32]
     lib.idb_gene_bitsets.restype = C.c_int
+    lib.idb_gene_bitsets_gpu.argtypes = [vp, u32, u32, vp, vp, vp, u32, vp, vp, vp, vp, u32]
+    lib.idb_gene_bitsets_gpu.restype = C.c_int
     lib.idb_dbscan.argtypes = [vp, C.POINTER(IdbDataset), C.POINTER(IdbParams), C.POINTER(IdbOutputs),
                                C.POINTER(IdbStats)]
     lib.idb_dbscan.restype = C.c_int

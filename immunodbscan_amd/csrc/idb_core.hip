@@ -42,7 +42,7 @@ enum Slot {
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
-    S_DIST, S_CNT2, S_EQG,
+    S_DIST, S_CNT2, S_EQG, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
     S_COUNT
 };
 
@@ -1578,6 +1578,81 @@ int idb_neighbour_pairs(idb_ctx *c, const idb_dataset *d, const idb_params *p, u
         return IDB_ERR_OOM;
     }
     return rc;
+}
+
+int idb_gene_bitsets_gpu(idb_ctx *c, uint32_t mem, uint32_t n, const uint8_t *calls, const uint64_t *call_off,
+                         const uint32_t *call_len, uint32_t n_genes, const uint8_t *genes, const uint64_t *gene_off,
+                         const uint32_t *gene_len, int32_t *out, uint32_t width)
+{
+    if (!c) return IDB_ERR_ARG;
+    c->err.clear();
+    if ((n && (!call_off || !call_len || !out)) || (n_genes && (!genes || !gene_off || !gene_len)) ||
+        (mem != IDB_MEM_HOST && mem != IDB_MEM_DEVICE)) {
+        c->err = "null or inconsistent argument";
+        return IDB_ERR_ARG;
+    }
+    if (width < (n_genes + 31) / 32) {
+        c->err = "width below ceil(n_genes / 32)";
+        return IDB_ERR_ARG;
+    }
+    if (width > IDB_GENE_MAX_WIDTH) {
+        c->err = "more than 256 bit words per row";
+        return IDB_ERR_UNSUPPORTED;
+    }
+    if (n == 0 || width == 0) return IDB_OK;
+    try {
+        check(c, hipSetDevice(c->device), "hipSetDevice");
+        hipStream_t s = c->stream;
+        // headers and 4-byte words of the names (names that cannot match get
+        // len = 0xFFFFFFFF: empty, or longer than the reference's 1024-thread
+        // launch allows, cuda/BitsetVector.inl:204)
+        std::vector<idb_gene_hdr> hdr(std::max(n_genes, 1u));
+        std::vector<uint32_t> words(1, 0u);
+        for (uint32_t g = 0; g < n_genes; ++g) {
+            const uint32_t gl = gene_len[g];
+            if (gl == 0 || gl > 1024) {
+                hdr[g] = {0u, 0u, 0xFFFFFFFFu, 0u};
+                continue;
+            }
+            const uint32_t woff = (uint32_t)words.size();
+            for (uint32_t k = 0; k < gl; k += 4) {
+                uint32_t v = 0;
+                for (uint32_t b = 0; b < 4 && k + b < gl; ++b) v |= (uint32_t)genes[gene_off[g] + k + b] << (8 * b);
+                words.push_back(v);
+            }
+            uint32_t last = 0;
+            for (uint32_t b = 0; b < 4 && b < gl; ++b) last |= (uint32_t)genes[gene_off[g] + gl - std::min(gl, 4u) + b] << (8 * b);
+            hdr[g] = {words[woff], last, gl, woff};
+        }
+        idb_gene_hdr *hd = buf<idb_gene_hdr>(c, S_GHDR, hdr.size());
+        uint32_t *wd = buf<uint32_t>(c, S_GWORDS, words.size());
+        check(c, hipMemcpyAsync(hd, hdr.data(), sizeof(idb_gene_hdr) * hdr.size(), hipMemcpyHostToDevice, s), "gene copy");
+        check(c, hipMemcpyAsync(wd, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice, s), "gene copy");
+        const uint8_t *cd = calls;
+        const uint64_t *od = call_off;
+        const uint32_t *ld = call_len;
+        int32_t *outd = out;
+        if (mem == IDB_MEM_HOST) {
+            uint64_t bytes = 0;
+            for (uint32_t i = 0; i < n; ++i) bytes = std::max<uint64_t>(bytes, call_off[i] + call_len[i]);
+            if (bytes && !calls) fail(c, IDB_ERR_ARG, "null call bytes");
+            cd = bytes ? stage_in<uint8_t>(c, S_GCALLS, calls, bytes, mem) : buf<uint8_t>(c, S_GCALLS, 4);
+            od = stage_in<uint64_t>(c, S_GOFF, call_off, n, mem);
+            ld = stage_in<uint32_t>(c, S_GLEN, call_len, n, mem);
+            outd = buf<int32_t>(c, S_GOUT, (size_t)n * width);
+        }
+        check(c, idb_launch_gene_bitsets(cd, od, ld, n, hd, wd, (uint32_t)words.size(), n_genes, outd, width, s),
+              "gene bitset launch");
+        if (mem == IDB_MEM_HOST)
+            check(c, hipMemcpyAsync(out, outd, sizeof(int32_t) * (size_t)n * width, hipMemcpyDeviceToHost, s),
+                  "bitset copy");
+        check(c, hipStreamSynchronize(s), "sync");
+    } catch (const Fail &f) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(c->stream);
+        return f.code;
+    }
+    return IDB_OK;
 }
 
 int idb_pair_distances(idb_ctx *c, const idb_dataset *d, const uint32_t *pairs, uint64_t n_pairs, uint32_t use_hamming,

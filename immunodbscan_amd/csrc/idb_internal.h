@@ -28,6 +28,15 @@
 
 #define IDB_NONE 0xFFFFFFFFu
 #define IDB_TILE 64               // entries per query / target block (one wave)
+// Gene bitsets on the device: one header per gene name (its first 4 bytes
+// and its last 4 bytes as little-endian words, zero-padded when shorter, its
+// length, the offset of its 4-byte words in the packed name array).  A name
+// that never matches (empty, or longer than 1024 bytes) has len = 0xFFFFFFFF.
+struct idb_gene_hdr {
+    uint32_t w0, w1, len, woff;
+};
+#define IDB_GENE_MAX_WIDTH 256u   // bit words per row (8,192 gene names)
+
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
@@ -113,6 +122,9 @@ hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s);
 hipError_t idb_launch_count_learned(const idb_pair_args &a, hipStream_t s);
 hipError_t idb_launch_pair_distances(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const uint2 *pairs,
                                      unsigned long long n, uint32_t W, uint32_t ham, uint32_t *dist, hipStream_t s);
+hipError_t idb_launch_gene_bitsets(const uint8_t *calls, const uint64_t *off, const uint32_t *len, uint32_t n,
+                                   const idb_gene_hdr *hdr, const uint32_t *gw, uint32_t n_words, uint32_t n_genes,
+                                   int32_t *out, uint32_t width, hipStream_t s);
 hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
                                   hipStream_t s);
 

@@ -54,6 +54,26 @@ def gene_bitsets(calls: Sequence, genes: Sequence) -> tuple[np.ndarray, int]:
     return out[:, :width], width
 
 
+def gene_bitsets_gpu(calls: Sequence, genes: Sequence, ctx: "_lib.Context | None" = None,
+                     width: int | None = None) -> tuple[np.ndarray, int]:
+    """R2 on the device through idb_gene_bitsets_gpu (host arrays in and out);
+    returns (words [n, width], width) like gene_bitsets."""
+    from .engine import default_context
+
+    ctx = ctx or default_context()
+    width = (len(genes) + 31) // 32 if width is None else width
+    n = len(calls)
+    out = np.zeros((n, max(width, 1)), np.int32)
+    cd, co, cl = pack_strings(calls)
+    gd, go, gl = pack_strings(genes)
+    cd = cd if len(cd) else np.zeros(1, np.uint8)
+    gd = gd if len(gd) else np.zeros(1, np.uint8)
+    ctx.check(_lib.lib.idb_gene_bitsets_gpu(ctx.handle, _lib.IDB_MEM_HOST, n, cd.ctypes.data, co.ctypes.data,
+                                            cl.ctypes.data, len(genes), gd.ctypes.data, go.ctypes.data,
+                                            gl.ctypes.data, out.ctypes.data, max(width, 1)))
+    return out[:, :width], width
+
+
 class InterlinkClass:
     """InterlinkClass(InputStrings, n_entries, V_genes_reference, V_gene_variant_count,
     J_genes_reference, J_gene_variant_count, threshold, minPts, use_only_hamming,

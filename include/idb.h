@@ -132,6 +132,17 @@ int idb_gene_bitsets(uint32_t n, const uint8_t *calls, const uint64_t *call_off,
                      uint32_t n_genes, const uint8_t *genes, const uint64_t *gene_off, const uint32_t *gene_len,
                      int32_t *out, uint32_t width);
 
+/* R2 gene bitsets on the device, with the same result as idb_gene_bitsets:
+ * one wave per call string tests every gene name.  Replaces
+ * bitset_container's constructor and its per-gene find_string_occurences
+ * launches (cuda/BitsetVector.inl:27-45, 118-209).  calls, call_off,
+ * call_len and out are in host memory (mem = IDB_MEM_HOST) or in device
+ * memory on the ctx device (IDB_MEM_DEVICE); the gene names are host arrays.
+ * width >= ceil(n_genes / 32) and at most 256 (IDB_ERR_UNSUPPORTED above). */
+int idb_gene_bitsets_gpu(idb_ctx *ctx, uint32_t mem, uint32_t n, const uint8_t *calls, const uint64_t *call_off,
+                         const uint32_t *call_len, uint32_t n_genes, const uint8_t *genes, const uint64_t *gene_off,
+                         const uint32_t *gene_len, int32_t *out, uint32_t width);
+
 /* One repertoire: duplicates, copies, core points, clusters and renumbering
  * (DBSCAN::doScan, cuda/PSDBSCAN.hcu:1300-1356, as driven by
  * InterlinkClass::InterlinkClass, cuda/Interlink.cu:34-95).  stats may be NULL. */
