@@ -117,6 +117,10 @@ class InterlinkClass:
         if rc != _lib.IDB_OK:
             raise RuntimeError(ctx.last_error())
         self._n_uniques = int(self.stats.n_uniques)
+        # the reference's run log (BenchmarkSingleton rows; run_log.write(path) gives its TSV)
+        from .runlog import RunLog, log_dbscan
+
+        self.run_log = log_dbscan(RunLog(), n, threshold, minPts, use_only_hamming, self.stats.as_dict(), eval_trees)
 
     # getters of cuda/Interlink.hpp:59-67
     def get_clusters_iterator(self) -> np.ndarray:

@@ -234,6 +234,11 @@ def test_interlink_python_boundary(gpu_ctx):
     assert il.get_v_bitset_width() == 4
     assert np.array_equal(il.v_gene_container_begin(), vb.reshape(-1))
     assert sum(il.stats.tree_depth) == len(triples)
+    # the run log carries the reference's messages with this run's numbers
+    txt = il.run_log.text()
+    assert f"Number of uniques = {exp['n_uniques']}" in txt
+    assert f"Total number of clusters = {int(exp['clone_id'].max())}" in txt
+    assert "Union find tree depth structure" in txt
 
 
 def test_device_resident_path(gpu_ctx):
