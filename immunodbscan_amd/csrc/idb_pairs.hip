@@ -199,10 +199,16 @@ __device__ __forceinline__ bool fuse_skipped(const idb_pair_args &a)
 // XCD, half of the tiles static) made the count pass slower, not faster.
 __device__ __forceinline__ uint32_t next_tile(uint32_t ti, uint32_t total_waves) { return ti + total_waves; }
 
-// The same, n_tiles past the end (no overflow near 2^32 tiles).
+// The Myers count pass deals the rounds back and forth: wave w takes
+// position w of even rounds and position waves - 1 - w of odd ones, so a wave
+// that drew from a heavy stretch of one round draws from the other end of
+// the next (3% off the 200k-row count pass).  n_tiles past the end.
 __device__ __forceinline__ uint32_t step_tile(uint32_t ti, uint32_t total_waves, uint32_t n_tiles)
 {
-    return ti < n_tiles && n_tiles - ti > total_waves ? ti + total_waves : n_tiles;
+    if (ti >= n_tiles) return n_tiles;
+    const uint32_t r = ti / total_waves, pos = ti % total_waves;
+    const uint64_t nx = (uint64_t)(r + 1) * total_waves + (total_waves - 1 - pos);
+    return nx < n_tiles ? (uint32_t)nx : n_tiles;
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; renumber them so that each XCD
