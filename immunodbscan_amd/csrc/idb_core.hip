@@ -1243,12 +1243,19 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             f.bmin = bmin;
             f.cnt = nullptr;
             f.edges = nullptr;
-            if (g) {
+            // The union or the fuse pass is chosen on the device (each kernel
+            // reads the chunk counter and returns when it is not the one to
+            // run), so no read-back waits between the count pass and the
+            // growth; the host learns which ran from the final statistics.
+            // Learned runs and callers of the pair list decide on the host.
+            if (g || !(learned || (x && x->pairs_out))) {
+                if (!g) check(c, hipEventRecord(c->kev[2], s), "event record");
                 check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
                 if (!learned) {
                     f.fuse_if_overflow = 1;
                     check(c, ham ? idb_launch_fuse_hamming(f, s) : idb_launch_fuse_myers(f, W, s), "fuse kernel launch");
                 }
+                if (!g) check(c, hipEventRecord(c->kev[3], s), "event record");
             } else {
                 uint32_t n_chunks = 0;
                 readback(c, &n_chunks, small + 12, sizeof(n_chunks));
@@ -1359,7 +1366,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
-    if (graph && E > 0) {
+    if (E > 0 && (graph || !(learned || (x && x->pairs_out)))) {
         from_edges = fin[12] <= a.max_chunks;
         if (learned && !from_edges)
             fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
