@@ -40,7 +40,7 @@ enum Slot {
     S_KEY, S_IDX, S_KEY2, S_IDX2, S_HEAD, S_REP, S_COPIES, S_CNT, S_CORE, S_PAR, S_BMIN,
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
-    S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_CMAP, S_TLUT_C, S_TLUT_F, S_HILEN, S_SMALL, S_WORK, S_TMP,
+    S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
     S_DIST, S_CNT2, S_EQG, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
     S_COUNT
@@ -52,7 +52,7 @@ struct idb_ctx {
     std::string err;
     void *ptr[S_COUNT] = {};
     size_t cap[S_COUNT] = {};
-    uint64_t *pinned = nullptr;   // small host staging area for counters
+    uint64_t *pinned = nullptr;   // pinned host staging: read-backs [0, 32 KiB), uploads [32, 64 KiB)
     hipEvent_t ev[IDB_N_STAGES + 1] = {};
     hipEvent_t kev[4] = {};       // around the count and the fuse kernels
     // autotuner: best count-kernel launch shape per kernel and shape class
@@ -829,10 +829,14 @@ void select_flagged(idb_ctx *c, size_t n, const uint8_t *flags, uint32_t *out, u
     check(c, rocprim::select(t, bytes, it, flags, out, count, n, c->stream), "select");
 }
 
+// Small read-backs go through the context's pinned staging area (a direct
+// copy instead of a staged one into pageable memory).
 void readback(idb_ctx *c, void *dst, const void *src, size_t bytes)
 {
-    check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "readback");
+    const bool staged = bytes <= IDB_STAGE_BYTES;
+    check(c, hipMemcpyAsync(staged ? (void *)c->pinned : dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "readback");
     check(c, hipStreamSynchronize(c->stream), "readback sync");
+    if (staged) std::memcpy(dst, c->pinned, bytes);
 }
 
 template <typename T>
@@ -1100,13 +1104,21 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         }
         hl[l] = (int16_t)std::min<uint32_t>(h, 32767u);
     }
-    int16_t *tlut_c = buf<int16_t>(c, S_TLUT_C, maxlen + 1), *tlut_f = buf<int16_t>(c, S_TLUT_F, maxlen + 1);
-    uint16_t *hilen = buf<uint16_t>(c, S_HILEN, maxlen + 1);
-    uint8_t *cmap = buf<uint8_t>(c, S_CMAP, 256);
-    check(c, hipMemcpyAsync(tlut_c, tc.data(), sizeof(int16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
-    check(c, hipMemcpyAsync(tlut_f, tf.data(), sizeof(int16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
-    check(c, hipMemcpyAsync(hilen, hl.data(), sizeof(uint16_t) * (maxlen + 1), hipMemcpyHostToDevice, s), "lut");
-    check(c, hipMemcpyAsync(cmap, cmap_h, 256, hipMemcpyHostToDevice, s), "cmap");
+    // the three length tables and the letter map in one upload from pinned
+    // memory: [count t | fuse t | band end] as int16 per length, then 256 codes
+    const size_t tl = sizeof(int16_t) * (maxlen + 1), lut_bytes = 3 * tl + 256;
+    uint8_t *lut = buf<uint8_t>(c, S_TLUT_C, lut_bytes);
+    int16_t *tlut_c = (int16_t *)lut, *tlut_f = (int16_t *)(lut + tl);
+    uint16_t *hilen = (uint16_t *)(lut + 2 * tl);
+    uint8_t *cmap = lut + 3 * tl;
+    {
+        uint8_t *up = (uint8_t *)c->pinned + IDB_STAGE_BYTES;  // maxlen <= 2048: 12.5 KiB at most
+        std::memcpy(up, tc.data(), tl);
+        std::memcpy(up + tl, tf.data(), tl);
+        std::memcpy(up + 2 * tl, hl.data(), tl);
+        std::memcpy(up + 3 * tl, cmap_h, 256);
+        check(c, hipMemcpyAsync(lut, up, lut_bytes, hipMemcpyHostToDevice, s), "lut");
+    }
 
     unsigned long long *work = buf<unsigned long long>(c, S_WORK, IDB_WORK_U64);
     check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "memset");
@@ -1439,7 +1451,7 @@ int idb_create(idb_ctx **out, int device)
     idb_ctx *c = new idb_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->pinned, 2 * IDB_STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
         return IDB_ERR_HIP;

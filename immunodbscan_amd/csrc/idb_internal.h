@@ -37,6 +37,7 @@ struct idb_gene_hdr {
 };
 #define IDB_GENE_MAX_WIDTH 256u   // bit words per row (8,192 gene names)
 
+#define IDB_STAGE_BYTES 32768u    // each half of a context's pinned staging area
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
