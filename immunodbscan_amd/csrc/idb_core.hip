@@ -37,7 +37,7 @@
 // Context and device arena
 enum Slot {
     S_SEQ, S_OFF, S_LEN, S_V, S_J, S_RID, S_RSTART,
-    S_KEY, S_IDX, S_KEY2, S_IDX2, S_HEAD, S_REP, S_COPIES, S_CNT, S_CORE, S_PAR, S_BMIN,
+    S_KEY, S_GSLOT, S_HTAB, S_REP, S_COPIES, S_CNT, S_CORE, S_PAR, S_BMIN,
     S_FLAG, S_UNIQ, S_NUM, S_MEMB, S_EOFF, S_EKEY, S_EROW, S_EKEY2, S_EROW2,
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
@@ -211,7 +211,7 @@ __global__ void k_fill_rid(const uint32_t *rstart, uint32_t n_reps, uint32_t n, 
 // 128 are collected in per-thread masks and merged once per wave.
 __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
                           const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t n, uint64_t hmask,
-                          uint64_t *key, uint32_t *idx, uint32_t *small)
+                          uint64_t *key, uint32_t *small)
 {
     __shared__ uint32_t sym[8];
     __shared__ uint32_t smax;
@@ -247,7 +247,6 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
         for (uint32_t w = 0; w < vw; ++w) h = mix64(h, (uint32_t)vb[(size_t)i * vw + w]);
         for (uint32_t w = 0; w < jw; ++w) h = mix64(h, 0x100000000ull | (uint32_t)jb[(size_t)i * jw + w]);
         key[i] = fin64(h) & hmask;
-        idx[i] = i;
         lmax = L > lmax ? L : lmax;
     }
 #pragma unroll
@@ -273,12 +272,6 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
     if (threadIdx.x == 0) atomicMax(&small[8], smax);
 }
 
-__global__ void k_seghead(const uint64_t *skey, uint32_t n, uint32_t *head)
-{
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
-        head[p] = (p == 0 || skey[p] != skey[p - 1]) ? p : 0u;
-}
-
 __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
                            const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t a, uint32_t b)
 {
@@ -299,33 +292,58 @@ __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32
     return true;
 }
 
-// R4: representative = minimum index among equal rows.  Within a hash run the
-// stable sort keeps rows ascending, so the first equal row is the minimum.
-__global__ void k_dedup(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
-                        const int32_t *jb, uint32_t jw, const uint32_t *rid, const uint32_t *sidx, const uint32_t *head,
-                        uint32_t n, uint32_t *rep, uint32_t *copies)
+__global__ void k_htab_clear(uint2 *tab, uint32_t P)
 {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t i = sidx[p];
-        uint32_t r = i;
-        for (uint32_t q = head[p]; q < p; ++q) {
-            const uint32_t j = sidx[q];
-            if (rows_equal(seq, off, len, vb, vw, jb, jw, rid, i, j)) {
-                r = j;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P; s += gridDim.x * blockDim.x)
+        tab[s] = make_uint2(IDB_NONE, 0u);
+}
+
+// R3/R4 with an open-addressing table of P >= 2N slots, one slot per group of
+// equal rows: x = the smallest row of the group seen so far (IDB_NONE while
+// the slot is free), y = the group's size.  A row starts at its hash's slot
+// and probes linearly.  A slot belongs to one group from the moment it is
+// claimed, and every member of a group is as good as any other to compare
+// with, so members lower x with atomicMin while others compare against it;
+// equal rows share a hash, hence a probe sequence, and all stop at the first
+// slot their group claimed.  At most N of the P slots fill, so every probe
+// ends at a free slot or at the row's group.
+__global__ void k_dedup_insert(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb,
+                               uint32_t vw, const int32_t *jb, uint32_t jw, const uint32_t *rid, const uint64_t *key,
+                               uint32_t n, uint2 *tab, uint32_t pmask, uint32_t *gslot)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t h = key[i];
+        uint32_t s = (uint32_t)(h ^ (h >> 32)) & pmask;
+        for (;;) {
+            uint32_t *x = &tab[s].x;
+            uint32_t o = __hip_atomic_load(x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o == IDB_NONE) {
+                o = atomicCAS(x, IDB_NONE, i);
+                if (o == IDB_NONE) break;  // claimed for a new group
+            }
+            if (key[o] == h && rows_equal(seq, off, len, vb, vw, jb, jw, rid, i, o)) {
+                atomicMin(x, i);
                 break;
             }
+            s = (s + 1) & pmask;
         }
-        rep[i] = r;
-        atomicAdd(copies + r, 1u);
+        atomicAdd(&tab[s].y, 1u);
+        gslot[i] = s;
     }
 }
 
-__global__ void k_after_dedup(const uint32_t *rep, const uint32_t *copies, uint32_t n, uint8_t *flag, int32_t *cnt,
-                              uint32_t *par, uint32_t *bmin, uint32_t *rflag)
+// Representatives, copies and the per-row state of the later stages: flag
+// (unique), cnt (own copies), par (own root), bmin (no core neighbour yet).
+__global__ void k_dedup_resolve(const uint2 *tab, const uint32_t *gslot, uint32_t n, uint32_t *rep, uint32_t *copies,
+                                uint8_t *flag, int32_t *cnt, uint32_t *par, uint32_t *bmin, uint32_t *rflag)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        flag[i] = rep[i] == i;
-        cnt[i] = (int32_t)copies[i];
+        const uint2 g = tab[gslot[i]];
+        const uint32_t cp = g.x == i ? g.y : 0u;
+        rep[i] = g.x;
+        copies[i] = cp;
+        flag[i] = g.x == i;
+        cnt[i] = (int32_t)cp;
         par[i] = i;
         bmin[i] = IDB_NONE;
         rflag[i] = 0u;
@@ -599,13 +617,15 @@ __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, con
 }
 
 // Sizes the host needs, gathered on the device so that one read-back brings
-// them: sz[0..1] = entries (eoff[U]); sz[2] = buckets, sz[3] = query blocks,
-// sz[4] = tiles.
+// them: sz[0..1] = entries (eoff[U], when there are bucket memberships),
+// sz[5] = uniques; later sz[2] = buckets, sz[3] = query blocks, sz[4] = tiles.
 __global__ void k_size_entries(const uint64_t *eoff, const uint32_t *num, uint32_t *sz)
 {
-    const uint64_t e = eoff[num[0]];
+    const uint32_t U = num[0];
+    const uint64_t e = eoff ? eoff[U] : 0u;
     sz[0] = (uint32_t)e;
     sz[1] = (uint32_t)(e >> 32);
+    sz[5] = U;
 }
 
 __global__ void k_size_tiles(const uint32_t *num, const uint32_t *qboff, const uint32_t *toff, uint32_t *sz)
@@ -812,14 +832,6 @@ void reserve_scan_tmp(idb_ctx *c, size_t n_plus_1)
     (void)tmp(c, bytes);
 }
 
-void max_scan(idb_ctx *c, const uint32_t *in, uint32_t *out, size_t n)
-{
-    size_t bytes = 0;
-    check(c, rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::maximum<uint32_t>(), c->stream), "max scan size");
-    void *t = tmp(c, bytes);
-    check(c, rocprim::inclusive_scan(t, bytes, in, out, n, rocprim::maximum<uint32_t>(), c->stream), "max scan");
-}
-
 void select_flagged(idb_ctx *c, size_t n, const uint8_t *flags, uint32_t *out, uint32_t *count)
 {
     rocprim::counting_iterator<uint32_t> it(0u);
@@ -982,15 +994,55 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     // --- Identify Duplicates
     uint32_t *small = buf<uint32_t>(c, S_SMALL, 128);
     check(c, hipMemsetAsync(small, 0, 128 * sizeof(uint32_t), s), "memset");
-    uint64_t *key = buf<uint64_t>(c, S_KEY, N), *key2 = buf<uint64_t>(c, S_KEY2, N);
-    uint32_t *idx = buf<uint32_t>(c, S_IDX, N), *idx2 = buf<uint32_t>(c, S_IDX2, N);
+    uint64_t *key = buf<uint64_t>(c, S_KEY, N);
     // 32 key bits are plenty to group duplicates (colliding rows are told apart
-    // by the byte comparison) and halve the sort passes of a 64-bit key
+    // by the byte comparison)
     const unsigned hbits = p->debug_hash_bits == 0 ? 32u : std::min(p->debug_hash_bits, 64u);
     const uint64_t hmask = hbits >= 64 ? ~0ull : ((1ull << hbits) - 1ull);
     hipLaunchKernelGGL(k_rowscan, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, N, hmask,
-                       key, idx, small);
-    uint32_t hs[16];
+                       key, small);
+    uint32_t P = 256;
+    while (P < 2ull * N) P <<= 1;
+    uint2 *htab = buf<uint2>(c, S_HTAB, P);
+    uint32_t *gslot = buf<uint32_t>(c, S_GSLOT, N);
+    hipLaunchKernelGGL(k_htab_clear, dim3(blocks_for(P)), dim3(256), 0, s, htab, P);
+    hipLaunchKernelGGL(k_dedup_insert, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, key, N,
+                       htab, P - 1, gslot);
+    uint32_t *rep = buf<uint32_t>(c, S_REP, N);
+    uint32_t *copies = buf<uint32_t>(c, S_COPIES, N);
+    uint8_t *flag = buf<uint8_t>(c, S_FLAG, N);
+    int32_t *cnt = buf<int32_t>(c, S_CNT, N);
+    uint32_t *par = buf<uint32_t>(c, S_PAR, N), *bmin = buf<uint32_t>(c, S_BMIN, N);
+    uint32_t *rflag = buf<uint32_t>(c, S_RFLAG, N + 1);
+    uint32_t *rank = buf<uint32_t>(c, S_RANK, N + 1);
+    hipLaunchKernelGGL(k_dedup_resolve, dim3(blocks_for(N)), dim3(256), 0, s, htab, gslot, N, rep, copies, flag, cnt, par,
+                       bmin, rflag);
+    record(c, 1);
+
+    // --- Determine copies: uniques
+    uint32_t *uniq = buf<uint32_t>(c, S_UNIQ, N);
+    uint32_t *num = buf<uint32_t>(c, S_NUM, 4);
+    select_flagged(c, N, flag, uniq, num);
+    record(c, 2);
+
+    // --- Identify core points: buckets, tiles, count pass
+    const uint32_t gj = 32 * std::max(jw, 1u);
+    const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
+    // Memberships are scanned over N + 1 slots (0 past the U uniques).  The
+    // letter set and longest row (small[0..8]), the entry count (sz[0..1])
+    // and U (sz[5]) come back in one read-back.
+    uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)N + 1);
+    uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)N + 1);
+    uint32_t *sz = small + 100;
+    uint32_t single = 0;
+    uint32_t E = 0, U = 0;
+    if (vw > 0 && jw > 0) {
+        hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)N + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb, jw,
+                           0u, N, memb);
+        exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
+    }
+    hipLaunchKernelGGL(k_size_entries, dim3(1), dim3(64), 0, s, (vw > 0 && jw > 0) ? eoff : nullptr, num, sz);
+    uint32_t hs[106];
     readback(c, hs, small, sizeof(hs));
     const uint32_t maxlen = hs[8];
     uint8_t cmap_h[256];
@@ -1021,52 +1073,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         fail(c, IDB_ERR_UNSUPPORTED, "the learned filter needs Levenshtein mode and rows of at most 32 letters");
     if (learned && idb_encoder_device(x->enc) != c->device)
         fail(c, IDB_ERR_ARG, "the encoder lives on another device");
-    sort_pairs_u64(c, key, key2, idx, idx2, N, hbits);
-    uint32_t *head0 = buf<uint32_t>(c, S_HEAD, N);
-    uint32_t *rep = buf<uint32_t>(c, S_REP, N);
-    uint32_t *copies = buf<uint32_t>(c, S_COPIES, N);
-    hipLaunchKernelGGL(k_seghead, dim3(blocks_for(N)), dim3(256), 0, s, key2, N, idx);  // idx reused as scratch
-    max_scan(c, idx, head0, N);
-    check(c, hipMemsetAsync(copies, 0, sizeof(uint32_t) * N, s), "memset");
-    hipLaunchKernelGGL(k_dedup, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, idx2, head0, N,
-                       rep, copies);
-    record(c, 1);
-
-    // --- Determine copies: uniques, count init
-    uint8_t *flag = buf<uint8_t>(c, S_FLAG, N);
-    int32_t *cnt = buf<int32_t>(c, S_CNT, N);
-    uint32_t *par = buf<uint32_t>(c, S_PAR, N), *bmin = buf<uint32_t>(c, S_BMIN, N);
-    uint32_t *rflag = buf<uint32_t>(c, S_RFLAG, N + 1);
-    uint32_t *rank = buf<uint32_t>(c, S_RANK, N + 1);
-    hipLaunchKernelGGL(k_after_dedup, dim3(blocks_for(N)), dim3(256), 0, s, rep, copies, N, flag, cnt, par, bmin, rflag);
-    uint32_t *uniq = buf<uint32_t>(c, S_UNIQ, N);
-    uint32_t *num = buf<uint32_t>(c, S_NUM, 4);
-    select_flagged(c, N, flag, uniq, num);
-    record(c, 2);
-
-    // --- Identify core points: buckets, tiles, count pass
-    const uint32_t gj = 32 * std::max(jw, 1u);
-    const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
-    // Memberships are scanned over N + 1 slots (0 past the U uniques), so U
-    // and the entry count come back together in one read-back.
-    uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)N + 1);
-    uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)N + 1);
-    uint32_t *sz = small + 100;
-    uint32_t single = 0;
-    uint32_t E = 0, U = 0;
-    if (vw > 0 && jw > 0) {
-        hipLaunchKernelGGL(k_membership, dim3(blocks_for((uint64_t)N + 1)), dim3(256), 0, s, uniq, num, vb, vw, jb, jw,
-                           0u, N, memb);
-        exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
-        hipLaunchKernelGGL(k_size_entries, dim3(1), dim3(64), 0, s, eoff, num, sz);
-    }
     {
-        uint32_t ue[3] = {0, 0, 0};
-        check(c, hipMemcpyAsync(ue, num, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "readback");
-        if (vw > 0 && jw > 0) check(c, hipMemcpyAsync(ue + 1, sz, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "readback");
-        check(c, hipStreamSynchronize(s), "readback sync");
-        U = ue[0];
-        uint64_t E64 = (uint64_t)ue[1] | ((uint64_t)ue[2] << 32);
+        U = hs[105];
+        uint64_t E64 = (uint64_t)hs[100] | ((uint64_t)hs[101] << 32);
         // With no V or no J genes the gate of R6 fails for every pair: no entries.
         // Rows with many genes are copied into many buckets; past 4x the
         // uniques one bucket per repertoire with an explicit gate is cheaper.
