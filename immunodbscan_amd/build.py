@@ -67,7 +67,7 @@ def build_interlink() -> str:
     repertoire from a binary file and writes clone ids, copies and cores."""
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(PKG, "interlink_demo")
-    _run([HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+    _run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           os.path.join(CSRC, "Interlink.cpp"), os.path.join(CSRC, "interlink_demo.cpp"),
           "-L" + PKG, "-lidb_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
