@@ -42,7 +42,7 @@ enum Slot {
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
-    S_DIST, S_CNT2, S_EQG, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
+    S_DIST, S_CNT2, S_EQG, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
     S_COUNT
 };
 
@@ -333,9 +333,11 @@ __global__ void k_dedup_insert(const uint8_t *seq, const uint64_t *off, const ui
 }
 
 // Representatives, copies and the per-row state of the later stages: flag
-// (unique), cnt (own copies), par (own root), bmin (no core neighbour yet).
+// (unique), cnt (own copies), par (own root), bmin (no core neighbour yet),
+// pe (no entry yet).
 __global__ void k_dedup_resolve(const uint2 *tab, const uint32_t *gslot, uint32_t n, uint32_t *rep, uint32_t *copies,
-                                uint8_t *flag, int32_t *cnt, uint32_t *par, uint32_t *bmin, uint32_t *rflag)
+                                uint8_t *flag, int32_t *cnt, uint32_t *par, uint32_t *bmin, uint32_t *rflag,
+                                uint32_t *pe)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint2 g = tab[gslot[i]];
@@ -347,6 +349,7 @@ __global__ void k_dedup_resolve(const uint2 *tab, const uint32_t *gslot, uint32_
         par[i] = i;
         bmin[i] = IDB_NONE;
         rflag[i] = 0u;
+        pe[i] = IDB_NONE;
     }
 }
 
@@ -430,7 +433,7 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
                                const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len, uint32_t lb,
                                uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag,
-                               uint32_t *eqg)
+                               uint32_t *eqg, uint32_t *pe, uint32_t *par_e, uint32_t *cmin_e)
 {
     __shared__ uint8_t cm[256];
     __shared__ uint32_t qgc[256 * IDB_QGRAM_WORDS];  // one 64-byte 3-gram count row per thread
@@ -442,6 +445,11 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         const uint32_t L = (uint32_t)(k & ((1ull << lb) - 1ull));
         const uint64_t bucket = k >> lb;
         elen[e] = (uint16_t)L;
+        // cluster growth in entry space: own root, no component minimum yet,
+        // and the row's first entry
+        par_e[e] = e;
+        cmin_e[e] = IDB_NONE;
+        atomicMin(pe + u, e);
         // a group is a bucket, or in Hamming mode a bucket's run of one length
         bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> lb) != bucket);
         idb_entry r;
@@ -680,12 +688,18 @@ __device__ __forceinline__ void block_add(uint32_t v, uint32_t *dst)
     }
 }
 
-__global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int32_t *core, uint32_t *small)
+// R8 core flags; a core row's first entry (pe) also carries the flag in bit 31.
+__global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int32_t *core, uint32_t *pe,
+                             uint32_t *small)
 {
     uint32_t local = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int32_t c = cnt[i] >= minpts;
         core[i] = c;
+        if (c) {
+            const uint32_t e = pe[i];
+            if (e != IDB_NONE) pe[i] = e | 0x80000000u;
+        }
         local += (uint32_t)c;
     }
     block_add(local, small + 9);
@@ -1015,8 +1029,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     uint32_t *par = buf<uint32_t>(c, S_PAR, N), *bmin = buf<uint32_t>(c, S_BMIN, N);
     uint32_t *rflag = buf<uint32_t>(c, S_RFLAG, N + 1);
     uint32_t *rank = buf<uint32_t>(c, S_RANK, N + 1);
+    uint32_t *pe = buf<uint32_t>(c, S_PE, N);
     hipLaunchKernelGGL(k_dedup_resolve, dim3(blocks_for(N)), dim3(256), 0, s, htab, gslot, N, rep, copies, flag, cnt, par,
-                       bmin, rflag);
+                       bmin, rflag, pe);
     record(c, 1);
 
     // --- Determine copies: uniques
@@ -1134,6 +1149,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     unsigned long long edge_cap = 0;
     uint32_t B = 0, T = 0;
     idb_pair_args a{};
+    idb_union_args ua{};
     if (E > 0) {
         uint64_t *ekey = buf<uint64_t>(c, S_EKEY, E), *ekey2 = buf<uint64_t>(c, S_EKEY2, E);
         uint32_t *erow = buf<uint32_t>(c, S_EROW, E), *erow2 = buf<uint32_t>(c, S_EROW2, E);
@@ -1159,9 +1175,16 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // Nucleotide rows: the 3-gram profile bound (learned runs use their own proposals).
         const bool qgram = !ham && !learned && K <= 4 && maxlen >= 16;
         uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
+        uint32_t *par_e = buf<uint32_t>(c, S_PARE, E), *cmin_e = buf<uint32_t>(c, S_CMINE, E);
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
-                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg);
+                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e);
+        ua.pe = pe;
+        ua.par_e = par_e;
+        ua.cmin_e = cmin_e;
+        ua.n_rows = N;
+        ua.par = par;
+        ua.bmin = bmin;
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
         // Buckets B <= E and query blocks QB <= E stay on the device: the
@@ -1252,7 +1275,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             else check(c, ham ? idb_launch_count_hamming(a, s) : idb_launch_count_myers(a, W, s), "count kernel launch");
             if (!g) check(c, hipEventRecord(c->kev[1], s), "event record");
         }
-        hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, small);
+        hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, pe,
+                           small);
         if (!g) record(c, 3);
 
         // --- Grow clusters: union over the recorded neighbour pairs, or a
@@ -1271,7 +1295,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             // Learned runs and callers of the pair list decide on the host.
             if (g || !(learned || (x && x->pairs_out))) {
                 if (!g) check(c, hipEventRecord(c->kev[2], s), "event record");
-                check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
+                check(c, idb_launch_union_edges(a, ua, s), "union kernel launch");
                 if (!learned) {
                     f.fuse_if_overflow = 1;
                     check(c, ham ? idb_launch_fuse_hamming(f, s) : idb_launch_fuse_myers(f, W, s), "fuse kernel launch");
@@ -1305,7 +1329,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
                     *x->n_pairs_out = k;
                 }
                 check(c, hipEventRecord(c->kev[2], s), "event record");
-                if (from_edges) check(c, idb_launch_union_edges(a, core, par, bmin, s), "union kernel launch");
+                if (from_edges) check(c, idb_launch_union_edges(a, ua, s), "union kernel launch");
                 else check(c, ham ? idb_launch_fuse_hamming(f, s) : idb_launch_fuse_myers(f, W, s), "fuse kernel launch");
                 check(c, hipEventRecord(c->kev[3], s), "event record");
             }

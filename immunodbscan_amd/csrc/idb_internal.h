@@ -126,8 +126,16 @@ hipError_t idb_launch_pair_distances(const uint8_t *seq, const uint64_t *off, co
 hipError_t idb_launch_gene_bitsets(const uint8_t *calls, const uint64_t *off, const uint32_t *len, uint32_t n,
                                    const idb_gene_hdr *hdr, const uint32_t *gw, uint32_t n_words, uint32_t n_genes,
                                    int32_t *out, uint32_t width, hipStream_t s);
-hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
-                                  hipStream_t s);
+// Cluster growth over the count pass's pairs (rewrites the pair list in place).
+struct idb_union_args {
+    const uint32_t *pe;           // per row: first entry | core flag << 31; IDB_NONE without entries
+    uint32_t *par_e;              // per entry: union-find parent, its own index on entry
+    uint32_t *cmin_e;             // per entry: smallest row of the component rooted there, IDB_NONE on entry
+    uint32_t n_rows;
+    uint32_t *par;                // per row: set to the component minimum for core rows
+    uint32_t *bmin;               // per row: border claims
+};
+hipError_t idb_launch_union_edges(const idb_pair_args &a, const idb_union_args &u, hipStream_t s);
 
 // Encoder (idb_learned.hip): bfloat16 embeddings of rows [0, n) on stream s.
 struct idb_encoder;

@@ -1108,12 +1108,30 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     }
 }
 
-// Union over the recorded pairs.  Parents are read through the caches and
-// never rewritten except by the linking exchange: a stale parent only points
-// to an earlier ancestor (parents only decrease), which ends in a failed
-// exchange that returns the current value, so every retry still makes
-// progress; without path-halving stores, hot roots see no write traffic
-// besides the links themselves.  Roots are compressed afterwards.
+// Cluster growth over the recorded pairs, in entry space.  The union-find
+// over rows touches two scattered parents per pair (rows are in input order),
+// and its speed is set by how many scattered cache lines the waves have in
+// flight.  A pair's two entries lie in one 64 x 64 tile, so the pairs of a
+// wave touch a few lines of an entry-indexed forest instead.  Four passes,
+// each returning at once when the pair list overflowed (the fuse pass over
+// the tiles grows the clusters then):
+//   1. pairs to entries: a pair of core rows becomes the pair of their first
+//      entries (pe, with the core flag in bit 31) and links the larger entry
+//      under the smaller with atomicMin, which never fails: each entry ends up
+//      under its smallest core neighbour, an acyclic forest of true links, so
+//      the exchanges of pass 2 meet far fewer contenders.  A core row with a
+//      non-core neighbour claims it (R10: the smallest core row wins).  Other
+//      pairs are marked dead.
+//   2. union over the entry pairs (exchange on roots, roots are minimum
+//      entries); a row is represented by its first entry only, so rows with
+//      several entries need no extra links;
+//   3. the smallest row of each component, collected at its root entry;
+//   4. each core row's parent becomes that row, so over rows the roots are
+//      the component minima, as the renumbering expects.
+// Parents are read through the caches and never rewritten except by the
+// linking exchange: a stale parent only points to an earlier ancestor
+// (parents only decrease), which ends in a failed exchange that returns the
+// current value, so every retry still makes progress.
 __device__ __forceinline__ uint32_t ld_cached(const uint32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1148,10 +1166,42 @@ __device__ void union_cached(uint32_t *par, uint32_t a, uint32_t b)
     }
 }
 
-// The chunk count is read on the device, so the launch does not wait for the
-// count pass on the host; after an overflow the kernel does nothing.
-__global__ void k_union_edges(const uint2 *edges, const uint32_t *fill, const uint32_t *n_chunks, uint32_t max_chunks,
-                              uint32_t chunk_log2, const int32_t *core, uint32_t *par, uint32_t *bmin)
+__device__ __forceinline__ bool pairs_overflowed(const uint32_t *n_chunks, uint32_t max_chunks)
+{
+    return *n_chunks > max_chunks;
+}
+
+constexpr uint32_t PE_CORE = 0x80000000u;
+
+__global__ void k_union_to_entries(uint2 *edges, const uint32_t *fill, const uint32_t *n_chunks, uint32_t max_chunks,
+                                   uint32_t chunk_log2, const uint32_t *pe, uint32_t *par_e, uint32_t *bmin)
+{
+    const uint32_t nc = *n_chunks;
+    if (nc > max_chunks) return;
+    const unsigned long long n = (unsigned long long)nc << chunk_log2;
+    const uint32_t cmask = (1u << chunk_log2) - 1u;
+    for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
+         e += (unsigned long long)gridDim.x * blockDim.x) {
+        if (((uint32_t)e & cmask) >= fill[e >> chunk_log2]) continue;
+        const uint2 p = edges[e];
+        const uint32_t px = pe[p.x], py = pe[p.y];
+        const bool cx = px != IDB_NONE && (px & PE_CORE) != 0u, cy = py != IDB_NONE && (py & PE_CORE) != 0u;
+        uint2 q = make_uint2(IDB_NONE, IDB_NONE);
+        if (cx && cy) {
+            q = make_uint2(px & ~PE_CORE, py & ~PE_CORE);
+            const uint32_t lo = q.x < q.y ? q.x : q.y, hi = q.x < q.y ? q.y : q.x;
+            if (lo != hi) atomicMin(par_e + hi, lo);
+        } else if (cx) {
+            atomicMin(bmin + p.y, p.x);
+        } else if (cy) {
+            atomicMin(bmin + p.x, p.y);
+        }
+        edges[e] = q;
+    }
+}
+
+__global__ void k_union_entries(const uint2 *edges, const uint32_t *fill, const uint32_t *n_chunks,
+                                uint32_t max_chunks, uint32_t chunk_log2, uint32_t *par_e)
 {
     const uint32_t nc = *n_chunks;
     if (nc > max_chunks) return;
@@ -1159,29 +1209,42 @@ __global__ void k_union_edges(const uint2 *edges, const uint32_t *fill, const ui
     const uint32_t cmask = (1u << chunk_log2) - 1u;
     // The grid-stride step is a multiple of 64 and chunks hold a multiple of
     // 64 pairs, so the lanes of a wave always hold consecutive slots of one
-    // chunk.  Consecutive pairs mostly share their first row x (one query's
-    // hits); a core-core run x-y1, x-y2, ... is joined as the chain x-y1,
-    // y1-y2, ...: the same component, without every lane queueing its
-    // exchange on the root of x.
+    // chunk.  Consecutive pairs mostly share their first entry x (one query's
+    // hits); a run x-y1, x-y2, ... is joined as the chain x-y1, y1-y2, ...:
+    // the same component, without every lane queueing its exchange on the
+    // root of x.
     const uint32_t lane = __lane_id();
     for (unsigned long long base = (blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x) & ~63ull; base < n;
          base += (unsigned long long)gridDim.x * blockDim.x) {
         const unsigned long long e = base + lane;
         const bool live = ((uint32_t)e & cmask) < fill[e >> chunk_log2];
         uint2 p = make_uint2(IDB_NONE, IDB_NONE);
-        int32_t cx = 0, cy = 0;
-        if (live) {
-            p = edges[e];
-            cx = core[p.x];
-            cy = core[p.y];
-        }
-        const bool cc = live && cx && cy;
+        if (live) p = edges[e];
+        const bool cc = p.x != IDB_NONE;
         // previous lane's pair, read with the whole wave active
         const uint32_t px = (uint32_t)__shfl_up((int)p.x, 1, 64), py = (uint32_t)__shfl_up((int)p.y, 1, 64);
         const bool pcc = __shfl_up((int)cc, 1, 64) != 0 && lane > 0;
-        if (cc) union_cached(par, (pcc && px == p.x) ? py : p.x, p.y);
-        else if (live && cx) atomicMin(bmin + p.y, p.x);
-        else if (live && cy) atomicMin(bmin + p.x, p.y);
+        if (cc) union_cached(par_e, (pcc && px == p.x) ? py : p.x, p.y);
+    }
+}
+
+__global__ void k_union_comp_min(const uint32_t *n_chunks, uint32_t max_chunks, const uint32_t *pe,
+                                 const uint32_t *par_e, uint32_t *cmin_e, uint32_t n_rows)
+{
+    if (pairs_overflowed(n_chunks, max_chunks)) return;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_rows; u += gridDim.x * blockDim.x) {
+        const uint32_t f = pe[u];
+        if (f != IDB_NONE && (f & PE_CORE)) atomicMin(cmin_e + find_cached(par_e, f & ~PE_CORE), u);
+    }
+}
+
+__global__ void k_union_to_rows(const uint32_t *n_chunks, uint32_t max_chunks, const uint32_t *pe,
+                                const uint32_t *par_e, const uint32_t *cmin_e, uint32_t n_rows, uint32_t *par)
+{
+    if (pairs_overflowed(n_chunks, max_chunks)) return;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_rows; u += gridDim.x * blockDim.x) {
+        const uint32_t f = pe[u];
+        if (f != IDB_NONE && (f & PE_CORE)) par[u] = cmin_e[find_cached(par_e, f & ~PE_CORE)];
     }
 }
 
@@ -1362,12 +1425,19 @@ hipError_t idb_launch_fuse_myers(const idb_pair_args &a, uint32_t W, hipStream_t
 hipError_t idb_launch_count_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<false>(a, s); }
 hipError_t idb_launch_fuse_hamming(const idb_pair_args &a, hipStream_t s) { return launch_hamming<true>(a, s); }
 
-hipError_t idb_launch_union_edges(const idb_pair_args &a, const int32_t *core, uint32_t *par, uint32_t *bmin,
-                                  hipStream_t s)
+hipError_t idb_launch_union_edges(const idb_pair_args &a, const idb_union_args &u, hipStream_t s)
 {
     if (!a.edges || a.max_chunks == 0) return hipSuccess;
-    const uint32_t blocks = resident_blocks((const void *)k_union_edges, 256, 0);
-    hipLaunchKernelGGL(k_union_edges, dim3(blocks), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
-                       a.chunk_log2, core, par, bmin);
+    const uint32_t b1 = resident_blocks((const void *)k_union_to_entries, 256, 0);
+    hipLaunchKernelGGL(k_union_to_entries, dim3(b1), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
+                       a.chunk_log2, u.pe, u.par_e, u.bmin);
+    const uint32_t b2 = resident_blocks((const void *)k_union_entries, 256, 0);
+    hipLaunchKernelGGL(k_union_entries, dim3(b2), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
+                       a.chunk_log2, u.par_e);
+    const uint32_t rb = (u.n_rows + 255) / 256 < 2048u ? (u.n_rows + 255) / 256 : 2048u;
+    hipLaunchKernelGGL(k_union_comp_min, dim3(rb ? rb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.pe, u.par_e,
+                       u.cmin_e, u.n_rows);
+    hipLaunchKernelGGL(k_union_to_rows, dim3(rb ? rb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.pe, u.par_e,
+                       u.cmin_e, u.n_rows, u.par);
     return hipGetLastError();
 }
