@@ -301,33 +301,33 @@ __global__ void k_htab_clear(uint2 *tab, uint32_t P)
 // R3/R4 with an open-addressing table of P >= 2N slots, one slot per group of
 // equal rows: x = the smallest row of the group seen so far (IDB_NONE while
 // the slot is free), y = the group's size.  A row starts at its hash's slot
-// and probes linearly.  A slot belongs to one group from the moment it is
-// claimed, and every member of a group is as good as any other to compare
-// with, so members lower x with atomicMin while others compare against it;
-// equal rows share a hash, hence a probe sequence, and all stop at the first
-// slot their group claimed.  At most N of the P slots fill, so every probe
-// ends at a free slot or at the row's group.
+// and probes linearly.  One 64-bit exchange claims a free slot with size 1
+// (most rows are unique and pay only that); a slot belongs to one group from
+// then on, and every member of a group is as good as any other to compare
+// with, so later members lower x with atomicMin and count themselves in y
+// while others compare against it.  Equal rows share a hash, hence a probe
+// sequence, and all stop at the first slot their group claimed.  At most N of
+// the P slots fill, so every probe ends at a free slot or at the row's group.
 __global__ void k_dedup_insert(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb,
                                uint32_t vw, const int32_t *jb, uint32_t jw, const uint32_t *rid, const uint64_t *key,
                                uint32_t n, uint2 *tab, uint32_t pmask, uint32_t *gslot)
 {
+    const unsigned long long FREE = (unsigned long long)IDB_NONE;  // x = IDB_NONE, y = 0
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint64_t h = key[i];
         uint32_t s = (uint32_t)(h ^ (h >> 32)) & pmask;
         for (;;) {
-            uint32_t *x = &tab[s].x;
-            uint32_t o = __hip_atomic_load(x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (o == IDB_NONE) {
-                o = atomicCAS(x, IDB_NONE, i);
-                if (o == IDB_NONE) break;  // claimed for a new group
-            }
+            const unsigned long long old =
+                atomicCAS((unsigned long long *)(tab + s), FREE, (1ull << 32) | (unsigned long long)i);
+            if (old == FREE) break;  // claimed for a new group
+            const uint32_t o = (uint32_t)old;
             if (key[o] == h && rows_equal(seq, off, len, vb, vw, jb, jw, rid, i, o)) {
-                atomicMin(x, i);
+                atomicMin(&tab[s].x, i);
+                atomicAdd(&tab[s].y, 1u);
                 break;
             }
             s = (s + 1) & pmask;
         }
-        atomicAdd(&tab[s].y, 1u);
         gslot[i] = s;
     }
 }
@@ -446,10 +446,11 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         const uint64_t bucket = k >> lb;
         elen[e] = (uint16_t)L;
         // cluster growth in entry space: own root, no component minimum yet,
-        // and the row's first entry
+        // and one entry to stand for the row (any of a row's entries serves:
+        // the components, and so the rows' roots, do not depend on which)
         par_e[e] = e;
         cmin_e[e] = IDB_NONE;
-        atomicMin(pe + u, e);
+        pe[u] = e;
         // a group is a bucket, or in Hamming mode a bucket's run of one length
         bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> lb) != bucket);
         idb_entry r;
@@ -688,7 +689,7 @@ __device__ __forceinline__ void block_add(uint32_t v, uint32_t *dst)
     }
 }
 
-// R8 core flags; a core row's first entry (pe) also carries the flag in bit 31.
+// R8 core flags; the entry that stands for a core row (pe) also carries the flag in bit 31.
 __global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int32_t *core, uint32_t *pe,
                              uint32_t *small)
 {
@@ -1182,6 +1183,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         ua.pe = pe;
         ua.par_e = par_e;
         ua.cmin_e = cmin_e;
+        ua.n_entries = E;
         ua.n_rows = N;
         ua.par = par;
         ua.bmin = bmin;

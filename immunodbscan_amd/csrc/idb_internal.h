@@ -128,10 +128,10 @@ hipError_t idb_launch_gene_bitsets(const uint8_t *calls, const uint64_t *off, co
                                    int32_t *out, uint32_t width, hipStream_t s);
 // Cluster growth over the count pass's pairs (rewrites the pair list in place).
 struct idb_union_args {
-    const uint32_t *pe;           // per row: first entry | core flag << 31; IDB_NONE without entries
+    const uint32_t *pe;           // per row: one of its entries | core flag << 31; IDB_NONE without entries
     uint32_t *par_e;              // per entry: union-find parent, its own index on entry
     uint32_t *cmin_e;             // per entry: smallest row of the component rooted there, IDB_NONE on entry
-    uint32_t n_rows;
+    uint32_t n_entries, n_rows;
     uint32_t *par;                // per row: set to the component minimum for core rows
     uint32_t *bmin;               // per row: border claims
 };

@@ -1112,19 +1112,19 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
 // over rows touches two scattered parents per pair (rows are in input order),
 // and its speed is set by how many scattered cache lines the waves have in
 // flight.  A pair's two entries lie in one 64 x 64 tile, so the pairs of a
-// wave touch a few lines of an entry-indexed forest instead.  Four passes,
+// wave touch a few lines of an entry-indexed forest instead.  Four steps,
 // each returning at once when the pair list overflowed (the fuse pass over
 // the tiles grows the clusters then):
-//   1. pairs to entries: a pair of core rows becomes the pair of their first
-//      entries (pe, with the core flag in bit 31) and links the larger entry
+//   1. pairs to entries: a pair of core rows becomes the pair of the entries
+//      that stand for them (pe, with the core flag in bit 31) and links the larger entry
 //      under the smaller with atomicMin, which never fails: each entry ends up
 //      under its smallest core neighbour, an acyclic forest of true links, so
-//      the exchanges of pass 2 meet far fewer contenders.  A core row with a
+//      the exchanges of step 2 meet far fewer contenders.  A core row with a
 //      non-core neighbour claims it (R10: the smallest core row wins).  Other
 //      pairs are marked dead.
-//   2. union over the entry pairs (exchange on roots, roots are minimum
-//      entries); a row is represented by its first entry only, so rows with
-//      several entries need no extra links;
+//   2. every entry pointed at its root, then the union over the entry pairs
+//      (exchange on roots, roots are minimum entries); a row is represented by
+//      one entry only, so rows with several entries need no extra links;
 //   3. the smallest row of each component, collected at its root entry;
 //   4. each core row's parent becomes that row, so over rows the roots are
 //      the component minima, as the renumbering expects.
@@ -1225,6 +1225,18 @@ __global__ void k_union_entries(const uint2 *edges, const uint32_t *fill, const 
         const uint32_t px = (uint32_t)__shfl_up((int)p.x, 1, 64), py = (uint32_t)__shfl_up((int)p.y, 1, 64);
         const bool pcc = __shfl_up((int)cc, 1, 64) != 0 && lane > 0;
         if (cc) union_cached(par_e, (pcc && px == p.x) ? py : p.x, p.y);
+    }
+}
+
+// Between the pre-link and the union: every entry points straight at its
+// tree's root, so the union's finds mostly take one step.  A concurrent find
+// reads the old parent or the root, both ancestors.
+__global__ void k_union_compress(const uint32_t *n_chunks, uint32_t max_chunks, uint32_t *par_e, uint32_t n_entries)
+{
+    if (pairs_overflowed(n_chunks, max_chunks)) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_entries; i += gridDim.x * blockDim.x) {
+        const uint32_t r = find_cached(par_e, i);
+        if (r != i) __hip_atomic_store(par_e + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1431,6 +1443,9 @@ hipError_t idb_launch_union_edges(const idb_pair_args &a, const idb_union_args &
     const uint32_t b1 = resident_blocks((const void *)k_union_to_entries, 256, 0);
     hipLaunchKernelGGL(k_union_to_entries, dim3(b1), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
                        a.chunk_log2, u.pe, u.par_e, u.bmin);
+    const uint32_t eb = (u.n_entries + 255) / 256 < 2048u ? (u.n_entries + 255) / 256 : 2048u;
+    hipLaunchKernelGGL(k_union_compress, dim3(eb ? eb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.par_e,
+                       u.n_entries);
     const uint32_t b2 = resident_blocks((const void *)k_union_entries, 256, 0);
     hipLaunchKernelGGL(k_union_entries, dim3(b2), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
                        a.chunk_log2, u.par_e);
