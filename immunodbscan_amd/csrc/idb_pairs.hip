@@ -1147,10 +1147,21 @@ __device__ __forceinline__ uint32_t find_cached(const uint32_t *par, uint32_t x)
     return x;
 }
 
+// Both roots at once: the two walks' loads are in flight together, so a
+// union waits for the longer walk, not for the sum of both.
+__device__ __forceinline__ void find_both(const uint32_t *par, uint32_t &a, uint32_t &b)
+{
+    for (;;) {
+        const uint32_t pa = ld_cached(par + a), pb = ld_cached(par + b);
+        if (pa == a && pb == b) return;
+        a = pa;
+        b = pb;
+    }
+}
+
 __device__ void union_cached(uint32_t *par, uint32_t a, uint32_t b)
 {
-    a = find_cached(par, a);
-    b = find_cached(par, b);
+    find_both(par, a, b);
     while (a != b) {
         if (a < b) {
             const uint32_t t = a;
@@ -1161,8 +1172,8 @@ __device__ void union_cached(uint32_t *par, uint32_t a, uint32_t b)
         if (__hip_atomic_compare_exchange_strong(par + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
             return;
-        a = find_cached(par, expected);
-        b = find_cached(par, b);
+        a = expected;
+        find_both(par, a, b);
     }
 }
 
