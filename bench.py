@@ -430,7 +430,7 @@ def main() -> int:
         "noise": s["n_noise"],
         "stage_ms": s["stage_ms"],
         "roofline": {
-            "kernel": f"k_pairs_myers<{W},false,{'true' if s['band_filter'] else 'false'}> (count pass)",
+            "kernel": f"k_pairs_myers<{W},false,{1 if s['band_filter'] else 0}> (count pass)",
             "bound": "valu",
             "achieved": achieved,
             "peak": VALU_PEAK_TOPS,
