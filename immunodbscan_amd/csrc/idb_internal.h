@@ -3,10 +3,12 @@
 //
 // Layout in HBM for one run (all arrays device-resident, row = input row):
 //   rows:     seq bytes, off[u64], len[u32], vbits[n*vw], jbits[n*jw]  (inputs)
-//             key[u64] / idx[u32]  duplicate hash and its sort permutation
+//             key[u64]  duplicate hash, gslot[u32] the row's duplicate-table slot
 //             rep[u32]  representative (min index of the duplicate class)
 //             copies[u32], cnt[i32]  class size / exact neighbour count
 //             par[u32]  union-find parent over rows, bmin[u32] border claim
+//             pe[u32]   the entry that stands for the row, core flag in bit 31
+//   duplicate table: P >= 2n slots of (smallest row, group size)
 //   entries:  one per (unique row, V gene, J gene) bucket membership, sorted by
 //             (repertoire, V gene, J gene, length, row); a bucket is a
 //             contiguous run.  Per entry a 32-byte record (idb_entry, loaded
@@ -19,7 +21,10 @@
 //             whose length band can hold a neighbour pair.
 //   edges:    neighbour pairs of the count pass in chunks of 2^chunk_log2
 //             pairs; a wave fills one chunk at a time, chunk_fill[c] says how
-//             many pairs chunk c holds.
+//             many pairs chunk c holds.  Cluster growth rewrites them in place
+//             as entry pairs.
+//   entry forest: par_e[u32] union-find parent per entry, cmin_e[u32] the
+//             smallest row of the component rooted at an entry.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
