@@ -42,7 +42,7 @@ enum Slot {
     S_ELEN, S_ECP, S_ECHK, S_ECODES, S_EPK, S_BSTART, S_QBOFF, S_TCNT, S_TOFF, S_TILES,
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
-    S_DIST, S_CNT2, S_EQG, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
+    S_DIST, S_CNT2, S_EQG, S_RSCNT, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
     S_COUNT
 };
 
@@ -1158,7 +1158,16 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         hipLaunchKernelGGL(k_emit_entries, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, eoff, vb, vw, jb, jw, len, rid,
                            single, gj, gvj, lb, ekey, erow);
         const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
-        sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, lb + bits_for(max_bucket));
+        const unsigned ebits = lb + bits_for(max_bucket);
+        bool sorted = false;
+        if (E <= IDB_SORT_SMALL_MAX && ebits <= 64) {
+            uint32_t *gc = buf<uint32_t>(c, S_RSCNT, (size_t)256 * idb_sort_small_blocks(E));
+            const hipError_t e = idb_sort_small(ekey, ekey2, erow, erow2, E, ebits, gc, s);
+            if (e == hipSuccess) sorted = true;
+            else if (e == hipErrorInvalidValue) (void)hipGetLastError();
+            else check(c, e, "entry sort");
+        }
+        if (!sorted) sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, ebits);
         uint16_t *elen = buf<uint16_t>(c, S_ELEN, E);
         idb_entry *ent = buf<idb_entry>(c, S_ENT, E);
         uint8_t *bflag = buf<uint8_t>(c, S_BFLAG, E);

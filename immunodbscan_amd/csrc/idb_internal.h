@@ -42,6 +42,16 @@ struct idb_gene_hdr {
 };
 #define IDB_GENE_MAX_WIDTH 256u   // bit words per row (8,192 gene names)
 
+// Entry order for runs of up to IDB_SORT_SMALL_MAX entries: stable radix
+// sort of (u64 key, u32 value) pairs with the keys' low end_bit bits, two
+// launches per digit pass (idb_sort.hip).  The sorted pairs end in (k1, v1);
+// gcnt holds 256 counters per block.  hipErrorInvalidValue: sizes this sort
+// does not take (nothing was launched).
+#define IDB_SORT_SMALL_MAX (1u << 18)
+uint32_t idb_sort_small_blocks(uint32_t n);
+hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, uint32_t n, uint32_t end_bit,
+                          uint32_t *gcnt, hipStream_t s);
+
 #define IDB_STAGE_BYTES 32768u    // each half of a context's pinned staging area
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
