@@ -207,12 +207,15 @@ __global__ void k_fill_rid(const uint32_t *rstart, uint32_t n_reps, uint32_t n, 
     }
 }
 
-// Row scan: content hash (R3 key), letter set, longest row.  Letters below
-// 128 are collected in per-thread masks and merged once per wave.
+// Row scan: content hash (R3 key), letter set, longest row; it also clears
+// the duplicate table's P slots.  Letters below 128 are collected in
+// per-thread masks and merged once per wave.
 __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
                           const int32_t *jb, uint32_t jw, const uint32_t *rid, uint32_t n, uint64_t hmask,
-                          uint64_t *key, uint32_t *small)
+                          uint64_t *key, uint32_t *small, uint2 *tab, uint32_t P)
 {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P; s += gridDim.x * blockDim.x)
+        tab[s] = make_uint2(IDB_NONE, 0u);
     __shared__ uint32_t sym[8];
     __shared__ uint32_t smax;
     if (threadIdx.x < 8) sym[threadIdx.x] = 0;
@@ -290,12 +293,6 @@ __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32
         if (x.x != y.x || x.y != y.y || x.z != y.z || x.w != y.w) return false;
     }
     return true;
-}
-
-__global__ void k_htab_clear(uint2 *tab, uint32_t P)
-{
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P; s += gridDim.x * blockDim.x)
-        tab[s] = make_uint2(IDB_NONE, 0u);
 }
 
 // R3/R4 with an open-addressing table of P >= 2N slots, one slot per group of
@@ -716,17 +713,13 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t x)
     return x;
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
-
 // R9/R10: label root of every unique row (its component root, its border
 // core's root, or none) and root flags for R12.
 __global__ void k_roots(const uint32_t *uniq, const uint32_t *num, const int32_t *core, const uint32_t *par,
-                        const uint32_t *bmin, uint32_t *lab, uint32_t *rflag)
+                        const uint32_t *bmin, uint32_t *lab, uint32_t *rflag, uint32_t n)
 {
     const uint32_t U = num[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) rflag[n] = 0u;  // the renumbering scan's last slot
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < U; k += gridDim.x * blockDim.x) {
         const uint32_t u = uniq[k];
         uint32_t l = IDB_NONE;
@@ -757,7 +750,7 @@ __global__ void k_tree_depth(const uint32_t *rep, const int32_t *core, const uin
 __global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_t *rank, const uint32_t *rid,
                          const uint32_t *rstart, const int32_t *core, const uint32_t *copies, const int32_t *cnt,
                          uint32_t n, uint32_t *cid, int32_t *core_out, uint32_t *cop_out, uint32_t *cnt_out,
-                         uint32_t *small)
+                         uint32_t *unq_out, uint32_t *small)
 {
     uint32_t noise = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -772,6 +765,7 @@ __global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_
         core_out[i] = core[i];
         cop_out[i] = copies[i];
         if (cnt_out) cnt_out[i] = (uint32_t)cnt[i];
+        unq_out[i] = 0u;  // the uniques' positions are written next
     }
     block_add(noise, small + 10);
 }
@@ -1014,13 +1008,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     // by the byte comparison)
     const unsigned hbits = p->debug_hash_bits == 0 ? 32u : std::min(p->debug_hash_bits, 64u);
     const uint64_t hmask = hbits >= 64 ? ~0ull : ((1ull << hbits) - 1ull);
-    hipLaunchKernelGGL(k_rowscan, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, N, hmask,
-                       key, small);
     uint32_t P = 256;
     while (P < 2ull * N) P <<= 1;
     uint2 *htab = buf<uint2>(c, S_HTAB, P);
     uint32_t *gslot = buf<uint32_t>(c, S_GSLOT, N);
-    hipLaunchKernelGGL(k_htab_clear, dim3(blocks_for(P)), dim3(256), 0, s, htab, P);
+    hipLaunchKernelGGL(k_rowscan, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, N, hmask,
+                       key, small, htab, P);
     hipLaunchKernelGGL(k_dedup_insert, dim3(blocks_for(N)), dim3(256), 0, s, seq, off, len, vb, vw, jb, jw, rid, key, N,
                        htab, P - 1, gslot);
     uint32_t *rep = buf<uint32_t>(c, S_REP, N);
@@ -1348,17 +1341,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         if (!g) record(c, 4);
 
         // --- Reduction of cluster information tree structure
-        hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag);
+        hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag, N);
         if (p->tree_stats)
             hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, small + 16);
         if (!g) record(c, 5);
 
         // --- Re-number clusters and identify noise
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, rflag + N, 1u, 0u);
         exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);
         hipLaunchKernelGGL(k_labels, dim3(blocks_for(N)), dim3(256), 0, s, rep, lab, rank, rid, rstart, core, copies, cnt,
-                           N, cid, core_o, cop_o, cnt_o, small);
-        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(N)), dim3(256), 0, s, unq_o, N, 0u);
+                           N, cid, core_o, cop_o, cnt_o, unq_o, small);
         if (n_reps > 0)
             hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart,
                                n_reps, ustart);
