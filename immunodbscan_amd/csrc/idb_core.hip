@@ -271,8 +271,10 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
         atomicMax(&smax, lmax);
     }
     __syncthreads();
-    if (threadIdx.x < 8 && sym[threadIdx.x]) atomicOr(&small[threadIdx.x], sym[threadIdx.x]);
-    if (threadIdx.x == 0) atomicMax(&small[8], smax);
+    // eight copies of the nine words, so that blocks do not queue on one word
+    uint32_t *dst = small + IDB_SMALL_ROWSCAN + 9 * (blockIdx.x & 7u);
+    if (threadIdx.x < 8 && sym[threadIdx.x]) atomicOr(dst + threadIdx.x, sym[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(dst + 8, smax);
 }
 
 __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
@@ -700,7 +702,7 @@ __global__ void k_core_flags(const int32_t *cnt, uint32_t n, int32_t minpts, int
         }
         local += (uint32_t)c;
     }
-    block_add(local, small + 9);
+    block_add(local, small + IDB_SMALL_CORES + (blockIdx.x & 15u));  // 16 counters
 }
 
 __device__ __forceinline__ uint32_t find_root(const uint32_t *par, uint32_t x)
@@ -767,7 +769,7 @@ __global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_
         if (cnt_out) cnt_out[i] = (uint32_t)cnt[i];
         unq_out[i] = 0u;  // the uniques' positions are written next
     }
-    block_add(noise, small + 10);
+    block_add(noise, small + IDB_SMALL_NOISE + (blockIdx.x & 15u));  // 16 counters
 }
 
 // First unique position of every repertoire (binary search over uniques).
@@ -1001,8 +1003,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     }
 
     // --- Identify Duplicates
-    uint32_t *small = buf<uint32_t>(c, S_SMALL, 128);
-    check(c, hipMemsetAsync(small, 0, 128 * sizeof(uint32_t), s), "memset");
+    uint32_t *small = buf<uint32_t>(c, S_SMALL, IDB_SMALL_WORDS);
+    check(c, hipMemsetAsync(small, 0, IDB_SMALL_WORDS * sizeof(uint32_t), s), "memset");
     uint64_t *key = buf<uint64_t>(c, S_KEY, N);
     // 32 key bits are plenty to group duplicates (colliding rows are told apart
     // by the byte comparison)
@@ -1038,8 +1040,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     const uint32_t gj = 32 * std::max(jw, 1u);
     const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
     // Memberships are scanned over N + 1 slots (0 past the U uniques).  The
-    // letter set and longest row (small[0..8]), the entry count (sz[0..1])
-    // and U (sz[5]) come back in one read-back.
+    // entry count (sz[0..1]), U (sz[5]) and the row scan's letter set and
+    // longest row (its 8 copies) come back in one read-back.
     uint64_t *memb = buf<uint64_t>(c, S_MEMB, (size_t)N + 1);
     uint64_t *eoff = buf<uint64_t>(c, S_EOFF, (size_t)N + 1);
     uint32_t *sz = small + 100;
@@ -1051,14 +1053,20 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
     }
     hipLaunchKernelGGL(k_size_entries, dim3(1), dim3(64), 0, s, (vw > 0 && jw > 0) ? eoff : nullptr, num, sz);
-    uint32_t hs[106];
+    uint32_t hs[IDB_SMALL_ROWSCAN + 72];
     readback(c, hs, small, sizeof(hs));
-    const uint32_t maxlen = hs[8];
+    uint32_t lw[9] = {};  // letter-set words and the longest row, over the row scan's 8 copies
+    for (uint32_t cpy = 0; cpy < 8; ++cpy)
+        for (uint32_t t = 0; t < 9; ++t) {
+            const uint32_t v = hs[IDB_SMALL_ROWSCAN + 9 * cpy + t];
+            lw[t] = t < 8 ? (lw[t] | v) : std::max(lw[t], v);
+        }
+    const uint32_t maxlen = lw[8];
     uint8_t cmap_h[256];
     uint32_t K = 0;
     for (uint32_t b = 0; b < 256; ++b) {
         cmap_h[b] = (uint8_t)K;
-        if (hs[b >> 5] >> (b & 31) & 1u) ++K;
+        if (lw[b >> 5] >> (b & 31) & 1u) ++K;
     }
     if (maxlen > 2048) fail(c, IDB_ERR_UNSUPPORTED, "sequences longer than 2048 letters are not supported");
     const bool ham = p->use_hamming != 0;
@@ -1408,7 +1416,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         us.resize(n_reps + 1);
         check(c, hipMemcpyAsync(us.data(), ustart, sizeof(uint32_t) * (n_reps + 1), hipMemcpyDeviceToHost, s), "ustart");
     }
-    uint32_t fin[128];
+    uint32_t fin[IDB_SMALL_WORDS];
     unsigned long long wk[8];
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
@@ -1422,8 +1430,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         for (uint32_t r = 0; r < n_reps; ++r) n_uniques_h[r] = us[r + 1] - us[r];
     if (st) {
         st->n_uniques = U;
-        st->n_cores = fin[9];
-        st->n_noise = fin[10];
+        st->n_cores = 0;
+        st->n_noise = 0;
+        for (uint32_t k = 0; k < 16; ++k) {
+            st->n_cores += fin[IDB_SMALL_CORES + k];
+            st->n_noise += fin[IDB_SMALL_NOISE + k];
+        }
         st->max_len = maxlen;
         st->alphabet_size = K;
         st->n_buckets = B;

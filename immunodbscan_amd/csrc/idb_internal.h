@@ -53,6 +53,15 @@ hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1
                           uint32_t *gcnt, hipStream_t s);
 
 #define IDB_STAGE_BYTES 32768u    // each half of a context's pinned staging area
+// The run's small device words (zeroed at the start of a run): [12] chunks
+// handed out, [16, 67) the depth histogram, [100, 106) sizes for the host,
+// then 8 copies of the row scan's 9 words (letter set, longest row) and 16
+// partial counts each of core and noise rows (copies keep blocks from
+// queueing their atomics on one word).
+#define IDB_SMALL_ROWSCAN 128u
+#define IDB_SMALL_CORES 200u
+#define IDB_SMALL_NOISE 216u
+#define IDB_SMALL_WORDS 256u
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
