@@ -908,7 +908,7 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
                     a.no_xcd_order = nx;
                     check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
                     check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-                    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
+                    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * IDB_WORK_COPIES * sizeof(unsigned long long), s), "tune reset");
                     // best of two runs per shape
                     float ms = 1e30f;
                     for (int rep = 0; rep < 2; ++rep) {
@@ -916,7 +916,7 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
                             check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s),
                                   "tune reset");
                             check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-                            check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
+                            check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * IDB_WORK_COPIES * sizeof(unsigned long long), s), "tune reset");
                         }
                         check(c, hipEventRecord(c->kev[0], s), "event record");
                         const hipError_t e = learned ? idb_launch_count_learned(a, s)
@@ -938,7 +938,7 @@ void autotune(idb_ctx *c, idb_pair_args &a, uint32_t N, uint32_t W, bool ham, bo
         best.trials = trials;
         check(c, hipMemcpyAsync(a.cnt, snap, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "tune reset");
         check(c, hipMemsetAsync(small + 12, 0, sizeof(uint32_t), s), "tune reset");
-        check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "tune reset");
+        check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * IDB_WORK_COPIES * sizeof(unsigned long long), s), "tune reset");
         it = c->tuned.emplace(key, best).first;
     }
     a.tune_wpb = it->second.wpb;
@@ -1146,8 +1146,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         check(c, hipMemcpyAsync(lut, up, lut_bytes, hipMemcpyHostToDevice, s), "lut");
     }
 
-    unsigned long long *work = buf<unsigned long long>(c, S_WORK, IDB_WORK_U64);
-    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * sizeof(unsigned long long), s), "memset");
+    unsigned long long *work = buf<unsigned long long>(c, S_WORK, IDB_WORK_U64 * IDB_WORK_COPIES);
+    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * IDB_WORK_COPIES * sizeof(unsigned long long), s), "memset");
     unsigned long long edge_cap = 0;
     uint32_t B = 0, T = 0;
     idb_pair_args a{};
@@ -1417,10 +1417,11 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         check(c, hipMemcpyAsync(us.data(), ustart, sizeof(uint32_t) * (n_reps + 1), hipMemcpyDeviceToHost, s), "ustart");
     }
     uint32_t fin[IDB_SMALL_WORDS];
-    unsigned long long wk[8];
+    unsigned long long wkc[IDB_WORK_U64 * IDB_WORK_COPIES], wk[IDB_WORK_U64] = {};
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
-    check(c, hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, s), "stats");
+    check(c, hipMemcpyAsync(wkc, work, sizeof(wkc), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
+    for (uint32_t k = 0; k < IDB_WORK_U64 * IDB_WORK_COPIES; ++k) wk[k % IDB_WORK_U64] += wkc[k];
     if (E > 0 && (graph || !(learned || (x && x->pairs_out)))) {
         from_edges = fin[12] <= a.max_chunks;
         if (learned && !from_edges)

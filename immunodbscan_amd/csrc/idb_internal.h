@@ -64,6 +64,10 @@ hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1
 #define IDB_SMALL_WORDS 256u
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
+// ... kept in IDB_WORK_COPIES copies (a kernel block adds into copy
+// blockIdx & (copies - 1)), so that the waves' closing atomics do not queue
+// on one word; readers sum the copies.
+#define IDB_WORK_COPIES 16u
 #define IDB_MAX_W 8               // Myers 32-bit words per pattern (<= 256 letters)
 #define IDB_MAX_HW 64             // Hamming packed words per row
 #define IDB_MAX_LEN 65535u

@@ -138,6 +138,12 @@ struct EdgeStage {
     uint32_t total;   // pairs this wave produced
 };
 
+// This block's copy of the run's statistics counters.
+__device__ __forceinline__ unsigned long long *work_copy(const idb_pair_args &a)
+{
+    return a.work + IDB_WORK_U64 * (blockIdx.x & (IDB_WORK_COPIES - 1u));
+}
+
 __device__ __forceinline__ void close_chunk(const idb_pair_args &a, const EdgeStage &st, uint32_t lane)
 {
     if (st.chunk != IDB_NONE && st.chunk < a.max_chunks && lane == 0) a.chunk_fill[st.chunk] = st.fill;
@@ -181,7 +187,7 @@ __device__ __forceinline__ void finish_edges(const idb_pair_args &a, EdgeStage &
     if (!a.edges) return;
     flush_edges(a, st, lane);
     close_chunk(a, st, lane);
-    if (lane == 0 && st.total && a.work) atomicAdd(a.work + 4, (unsigned long long)st.total);
+    if (lane == 0 && st.total && a.work) atomicAdd(work_copy(a) + 4, (unsigned long long)st.total);
 }
 
 // Fuse pass launched behind the union step without a host decision: it runs
@@ -748,12 +754,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (!FUSE) finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
     if (lane == 0 && a.work) {
-        atomicAdd(a.work + (FUSE ? 2 : 0), gated);
+        atomicAdd(work_copy(a) + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
-            atomicAdd(a.work + 1, hits);
-            atomicAdd(a.work + 3, gletters);
-            atomicAdd(a.work + 5, mpairs);
-            atomicAdd(a.work + 6, mletters);
+            atomicAdd(work_copy(a) + 1, hits);
+            atomicAdd(work_copy(a) + 3, gletters);
+            atomicAdd(work_copy(a) + 5, mpairs);
+            atomicAdd(work_copy(a) + 6, mletters);
         }
     }
 }
@@ -939,11 +945,11 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
     gletters = wave_sum_u64(gletters);
     mletters = wave_sum_u64(mletters);
     if (lane == 0 && a.work) {
-        atomicAdd(a.work + 0, gated);
-        atomicAdd(a.work + 1, hits);
-        atomicAdd(a.work + 3, gletters);
-        atomicAdd(a.work + 5, mpairs);
-        atomicAdd(a.work + 6, mletters);
+        atomicAdd(work_copy(a) + 0, gated);
+        atomicAdd(work_copy(a) + 1, hits);
+        atomicAdd(work_copy(a) + 3, gletters);
+        atomicAdd(work_copy(a) + 5, mpairs);
+        atomicAdd(work_copy(a) + 6, mletters);
     }
 }
 
@@ -1100,10 +1106,10 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     }
     if (!FUSE) finish_edges(a, est, lane);
     if (lane == 0 && a.work) {
-        atomicAdd(a.work + (FUSE ? 2 : 0), gated);
+        atomicAdd(work_copy(a) + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
-            atomicAdd(a.work + 1, hits);
-            atomicAdd(a.work + 3, gletters);
+            atomicAdd(work_copy(a) + 1, hits);
+            atomicAdd(work_copy(a) + 3, gletters);
         }
     }
 }
