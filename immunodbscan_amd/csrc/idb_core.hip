@@ -738,14 +738,21 @@ __global__ void k_roots(const uint32_t *uniq, const uint32_t *num, const int32_t
 __global__ void k_tree_depth(const uint32_t *rep, const int32_t *core, const uint32_t *par, const uint32_t *bmin,
                              uint32_t n, uint32_t *hist)
 {
+    // a block histogram in LDS, added to the run's histogram once per bin
+    __shared__ uint32_t h[51];
+    for (uint32_t k = threadIdx.x; k < 51; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t d = 0, x = i;
         if (rep[x] != x) { x = rep[x]; ++d; }
         if (!core[x] && bmin[x] != IDB_NONE) { x = bmin[x]; ++d; }
         if (core[x])
             while (par[x] != x && d < 50) { x = par[x]; ++d; }
-        atomicAdd(hist + (d > 50 ? 50 : d), 1u);
+        atomicAdd(&h[d > 50 ? 50 : d], 1u);
     }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 51; k += blockDim.x)
+        if (h[k]) atomicAdd(hist + k, h[k]);
 }
 
 // R11/R12 labels plus the per-row outputs.

@@ -1335,9 +1335,19 @@ hipError_t launch_myers_f(const idb_pair_args &a, uint32_t W, dim3 block, size_t
 }
 
 template <bool FUSE>
-hipError_t launch_myers(const idb_pair_args &a, uint32_t W, hipStream_t s)
+hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
 {
-    if (a.n_tiles == 0) return hipSuccess;
+    if (a_in.n_tiles == 0) return hipSuccess;
+    // One-word rows with the letter bound (protein CDR3s): unless a shape was
+    // tuned, one wave per block, 4x the resident grid and plain block order,
+    // the autotuner's pick on 200k rows (0.276 against 0.305 ms for one
+    // resident round of 4-wave blocks in XCD order).
+    idb_pair_args a = a_in;
+    if (W == 1 && a.band_filter && a.tune_wpb == 0 && a.tune_grid_mul == 0) {
+        a.tune_wpb = 1;
+        a.tune_grid_mul = 4;
+        a.no_xcd_order = 1;
+    }
     const size_t per_wave = idb_myers_lds_per_wave(a.K, W) + (a.qgram_filter ? idb_qgram_lds_per_wave() : 0);
     const size_t limit = idb_lds_limit();
     uint32_t wpb = (uint32_t)(std::min<size_t>(limit, 65536) / per_wave);
