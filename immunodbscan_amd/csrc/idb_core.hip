@@ -1248,10 +1248,13 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.work = work;
         // Neighbour pairs found by the count pass are kept for the fuse step;
         // if they overflow the list, the fuse pass re-evaluates the tiles.
-        // Waves own chunks of 256 pairs (64 for small requested capacities).
+        // Waves own chunks of 256 pairs (64 for small requested capacities,
+        // 1,024 for lists of 16M pairs or more: every new chunk is one atomic
+        // on one counter, and on the 512-repertoire batch 4x fewer claims
+        // took the count pass from 3.40 to 3.15 ms).
         edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 64ull * E);
         edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 31);
-        a.chunk_log2 = edge_cap >= (1ull << 20) ? 8u : 6u;
+        a.chunk_log2 = edge_cap >= (1ull << 24) ? 10u : (edge_cap >= (1ull << 20) ? 8u : 6u);
         a.max_chunks = (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
         edge_cap = (unsigned long long)a.max_chunks << a.chunk_log2;
         a.edges = buf<uint2>(c, S_EDGES, edge_cap);
