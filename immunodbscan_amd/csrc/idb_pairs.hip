@@ -1338,14 +1338,16 @@ template <bool FUSE>
 hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
 {
     if (a_in.n_tiles == 0) return hipSuccess;
-    // One-word rows with the letter bound (protein CDR3s): unless a shape was
-    // tuned, one wave per block, 4x the resident grid and plain block order,
-    // the autotuner's pick on 200k rows (0.276 against 0.305 ms for one
-    // resident round of 4-wave blocks in XCD order).
+    // Unless a shape was tuned: one wave per block, plain block order, and a
+    // grid of 4x (one-word rows) or 8x (longer rows) the resident waves, so
+    // that waves done early are replaced while heavy tiles run.  These are
+    // the autotuner's picks on 200k protein rows (0.276 against 0.305 ms for
+    // one resident round of 4-wave blocks in XCD order) and on 20M
+    // nucleotide rows (1.10 against 1.31 s).
     idb_pair_args a = a_in;
-    if (W == 1 && a.band_filter && a.tune_wpb == 0 && a.tune_grid_mul == 0) {
+    if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
         a.tune_wpb = 1;
-        a.tune_grid_mul = 4;
+        a.tune_grid_mul = W == 1 ? 4u : 8u;
         a.no_xcd_order = 1;
     }
     const size_t per_wave = idb_myers_lds_per_wave(a.K, W) + (a.qgram_filter ? idb_qgram_lds_per_wave() : 0);
@@ -1366,8 +1368,16 @@ hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
 }
 
 template <int HWMAX, int BITS, bool FUSE>
-hipError_t launch_hamming_hw(const idb_pair_args &a, hipStream_t s)
+hipError_t launch_hamming_hw(const idb_pair_args &a_in, hipStream_t s)
 {
+    // Unless a shape was tuned: one wave per block, 4x the resident grid,
+    // plain block order (the autotuner's pick on the 512-repertoire batch).
+    idb_pair_args a = a_in;
+    if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
+        a.tune_wpb = 1;
+        a.tune_grid_mul = 4;
+        a.no_xcd_order = 1;
+    }
     const uint32_t wpb = pick_wpb(a, 4);
     const void *k = (const void *)k_pairs_hamming<HWMAX, BITS, FUSE>;
     const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, 64 * wpb, 0), a.tune_grid_mul));
