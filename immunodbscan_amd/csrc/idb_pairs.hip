@@ -1404,9 +1404,16 @@ hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
     return a.bits == 2 ? launch_hamming_bits<2, FUSE>(a, s) : launch_hamming_bits<8, FUSE>(a, s);
 }
 
-hipError_t launch_learned(const idb_pair_args &a, hipStream_t s)
+hipError_t launch_learned(const idb_pair_args &a_in, hipStream_t s)
 {
-    if (a.n_tiles == 0) return hipSuccess;
+    if (a_in.n_tiles == 0) return hipSuccess;
+    // the exact one-word pass's default shape, unless one was tuned
+    idb_pair_args a = a_in;
+    if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
+        a.tune_wpb = 1;
+        a.tune_grid_mul = 4;
+        a.no_xcd_order = 1;
+    }
     const size_t per_wave = idb_myers_lds_per_wave(a.K, 1);
     uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
     if (wpb > 4) wpb = 4;
