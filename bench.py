@@ -5,16 +5,21 @@ Workload (default): one repertoire of 200,000 amino-acid CDR3s from 20,000
 seed clonotypes (lognormal clone sizes, 1-3 edits per member, lengths
 N(15, 3) clipped to 8-25), 64 V genes (Zipf 1.1, second call p=0.2) and 8 J
 genes; neighbours are pairs with a common V and J gene and Levenshtein
-distance <= 2; minPts = 3.  A step is one full clustering call (duplicates,
-core points, clusters, labels) with the rows already resident in HBM and the
-labels left in HBM.
+distance <= 2; minPts = 3.  A step is one full clustering call as the
+reference's boundary makes it (InterlinkClass): rows in host memory at entry,
+copied to the device, clustered (duplicates, core points, clusters, labels),
+and clone ids, core flags, copies and uniques copied back to host memory.
+The same call with rows already in HBM and labels left there is reported as
+device_resident.
 
 Metric: pairs/sec under the reference's all-pairs work model, N(N-1)/2 per
 step divided by the step time; seqs clustered/sec is reported beside it.
 The CPU baseline is the oracle's literal all-pairs restatement of the
-reference, run on this host on the first rows of the same workload.
+reference, run on every core this process may use, on the whole workload.
 
 Beside the headline exact line the JSON carries:
+  c1        the 2,000-row Hamming workload, GPU and the literal CPU oracle
+            (median of 3) side by side;
   learned   the learned candidate filter on the same workload and on a
             2,000,000-row repertoire of the same generator: the encoder is
             trained on the device (untimed) on pairs labelled by the exact
@@ -64,14 +69,21 @@ QGRAM_OPS_PER_PAIR = 18
 
 
 def count_kernel_ops(s: dict, vw: int, jw: int) -> int:
+    """The count pass's algorithmic work as the work model defines it: the
+    V/J gate and length test per gated pair, Myers ops per text letter."""
     W = max(1, (s["max_len"] + 31) // 32)
     gate_ops = 2 * (vw + jw) + 2
-    ops = s["gated_pairs"] * gate_ops + s["myers_letters"] * myers_ops_per_letter(W)
+    return s["gated_pairs"] * gate_ops + s["myers_letters"] * myers_ops_per_letter(W)
+
+
+def filter_ops(s: dict) -> int:
+    """Ops of the exact lower bound that runs before Myers (absent letters
+    or 3-grams), which the work model does not count."""
     if s["band_filter"] == 1:
-        ops += s["gated_pairs"] * FILTER_OPS_PER_PAIR
-    elif s["band_filter"] == 2:
-        ops += s["gated_pairs"] * QGRAM_OPS_PER_PAIR
-    return ops
+        return s["gated_pairs"] * FILTER_OPS_PER_PAIR
+    if s["band_filter"] == 2:
+        return s["gated_pairs"] * QGRAM_OPS_PER_PAIR
+    return 0
 
 
 ENC_FLOPS_PER_ROW = 2 * 32 * (5 * 32 * 128 + 2 * 5 * 128 * 128)  # three conv layers, 32 positions
@@ -103,7 +115,7 @@ def learned_section(args, cl, exact_rep, exact_ms, exact_res) -> dict:
 
     t0 = time.perf_counter()
     vocab = learned.vocab_for(synth.LETTERS[synth.AA])
-    train = synth.generate(configs.c2_params(n_rows=200_000, n_seeds=20_000, seed=41))
+    train = synth.generate(configs.c2_params(n_rows=1_000_000, n_seeds=100_000, seed=41))
     pairs, labels = learned.training_pairs(train, cl, args.train_pairs, seed=3)
     enc = learned.Encoder(vocab, seed=2)
     tp = learned.TrainParams(epochs=args.train_epochs, batch_pairs=4096, lr=5e-4, margin=0.9)
@@ -162,7 +174,21 @@ def learned_section(args, cl, exact_rep, exact_ms, exact_res) -> dict:
     return out
 
 
-def batched_section(args) -> dict:
+def cpu_literal(reps, cl, threads: int) -> tuple[float, float]:
+    """Seconds and all-pairs count of the oracle's literal clustering of each
+    repertoire in turn, on `threads` host threads."""
+    from oracle import oracle as O
+
+    prm = O.params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, nthreads=threads)
+    t0 = time.perf_counter()
+    pairs = 0.0
+    for r in reps:
+        O.dbscan(O.Data.of(r), prm, fast=False)
+        pairs += r.n * (r.n - 1) / 2
+    return time.perf_counter() - t0, pairs
+
+
+def batched_section(args, threads: int) -> dict:
     from immunodbscan_amd import _lib, configs, engine, synth
 
     reps = configs.c3_repertoires(n_reps=args.c3_reps)
@@ -173,6 +199,17 @@ def batched_section(args) -> dict:
     s = st.as_dict()
     pairs = float(sum(r.n * (r.n - 1) / 2 for r in reps))
     log(f"batched: {len(reps)} repertoires, {cat.n} rows, {ms:.3f} ms/step")
+    # CPU: the literal oracle over the first repertoires, in order, until
+    # about 10 s of work; pairs/s extrapolates to the batch
+    from oracle import oracle as O  # noqa: F401  (checker only)
+
+    done, secs, cpairs = 0, 0.0, 0.0
+    while done < len(reps) and secs < 10.0:
+        dt, p = cpu_literal(reps[done:done + 8], configs.C3, threads)
+        secs += dt
+        cpairs += p
+        done += min(8, len(reps) - done)
+    cpu_v = cpairs / secs
     return {
         "workload": f"{len(reps)} nt repertoires, lognormal sizes (median 5k, 200-200k), Hamming eps 0.15, minPts 2",
         "repertoires": len(reps),
@@ -188,6 +225,10 @@ def batched_section(args) -> dict:
         "tuned_launch": {"waves_per_block": s["tuned_wpb"], "grid_x_resident": s["tuned_grid_mul"],
                          "xcd_order": s["tuned_xcd_order"] & 1, "candidates_timed": s["tuned_trials"]},
         "stage_ms": s["stage_ms"],
+        "cpu_baseline": {"value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
+                         "sample": f"literal all-pairs clustering of the first {done} repertoires "
+                                   f"({sum(r.n for r in reps[:done])} rows) in {secs:.1f} s; extrapolated"},
+        "gpu_over_cpu": (pairs / (ms * 1e-3)) / cpu_v,
     }
 
 
@@ -270,7 +311,9 @@ def pmc_traffic(rows: int):
         return float(json.load(f)["traffic_bytes"])
 
 
-def large_section(args) -> dict:
+def large_section(args, threads: int) -> dict:
+    import numpy as np
+
     from immunodbscan_amd import configs, engine, synth
 
     rep = synth.generate(configs.c5_params(n_rows=args.c5_rows, n_seeds=max(1, args.c5_rows // 200)))
@@ -283,6 +326,10 @@ def large_section(args) -> dict:
     ops = count_kernel_ops(s, rep.vw, rep.jw)
     kms = s["count_kernel_ms"]
     log(f"large: {ms:.1f} ms/step")
+    # CPU: the literal oracle on the first rows of the same workload
+    sub_n = min(n, args.c5_cpu_rows)
+    dt, cpairs = cpu_literal([rep.subset(np.arange(sub_n))], configs.C5, threads)
+    cpu_v = cpairs / dt
     return {
         "workload": f"{n} nt CDR3s, 24-150 nt, t=3 edits + V/J gate, minPts=3",
         "rows": n,
@@ -298,7 +345,11 @@ def large_section(args) -> dict:
         "stage_ms": s["stage_ms"],
         "count_kernel_roofline": {"bound": "valu", "achieved": ops / (kms * 1e-3) / 1e12, "peak": VALU_PEAK_TOPS,
                                   "unit": "TOP/s", "frac": ops / (kms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
-                                  "kernel_ms": kms, "algorithmic_ops": ops},
+                                  "kernel_ms": kms, "algorithmic_ops": ops, "filter_ops": filter_ops(s)},
+        "cpu_baseline": {"value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
+                         "sample": f"literal all-pairs clustering of the first {sub_n} rows in {dt:.1f} s; "
+                                   "extrapolated"},
+        "gpu_over_cpu": (n * (n - 1) / 2 / (ms * 1e-3)) / cpu_v,
     }
 
 
@@ -325,6 +376,50 @@ def cpu_baseline(rep, cl, rows: int, threads: int) -> dict:
     }
 
 
+def host_cores() -> int:
+    """Cores this process may run on (its affinity mask)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def c1_section(args, threads: int) -> dict:
+    """The 2,000-row Hamming workload (the reference CPU path's own config):
+    the GPU call from host memory to labels in host memory, and the oracle's
+    literal all-pairs clustering on every host core, median of 3 runs."""
+    import numpy as np
+
+    from immunodbscan_amd import configs, engine, synth
+    from oracle import oracle as O
+
+    rep = synth.generate(configs.c1_params())
+    cl = configs.C1
+    run = engine.HostRun(rep, cl)
+    ms, _ = timed(run, 50, 5)
+    n = rep.n
+    d = O.Data.of(rep)
+    prm = O.params(cl.eps, cl.abs_k, cl.min_pts, cl.use_hamming, nthreads=threads)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.dbscan(d, prm, fast=False)
+        ts.append(time.perf_counter() - t0)
+    cpu = float(np.median(ts))
+    pairs = n * (n - 1) / 2
+    return {
+        "workload": f"{n} nt CDR3s, Hamming, eps 0.15 relative, minPts 2 (host memory in and out)",
+        "rows": n,
+        "ms_per_step": ms,
+        "pairs_per_sec": pairs / (ms * 1e-3),
+        "seqs_clustered_per_sec": n / (ms * 1e-3),
+        "cpu_baseline": {"value": pairs / cpu, "unit": "pairs/s", "cores": threads, "kind": "port",
+                         "sample": f"the whole workload, literal all-pairs clustering, median of 3 runs "
+                                   f"({', '.join(f'{t * 1e3:.1f}' for t in ts)} ms)"},
+        "gpu_over_cpu": (pairs / (ms * 1e-3)) / (pairs / cpu),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,15 +427,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=200_000)
     ap.add_argument("--cpu-rows", type=int, default=200_000)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may use")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--train-pairs", type=int, default=1_000_000)
-    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--train-pairs", type=int, default=5_000_000)
+    ap.add_argument("--train-epochs", type=int, default=4)
     ap.add_argument("--c4-rows", type=int, default=2_000_000, help="0 skips the 2M-row learned comparison")
     ap.add_argument("--c3-reps", type=int, default=512)
     ap.add_argument("--c5-rows", type=int, default=20_000_000, help="0 skips the 20M-row nucleotide line")
+    ap.add_argument("--c5-cpu-rows", type=int, default=40_000)
     ap.add_argument("--gene-rows", type=int, default=10_000_000, help="0 skips the device gene-bitset line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB "
@@ -362,6 +459,7 @@ def main() -> int:
 
     from immunodbscan_amd import configs, engine, hip, synth
 
+    threads = args.cpu_threads or host_cores()
     cl = configs.C2
     rep = synth.generate(configs.c2_params(n_rows=args.rows, n_seeds=max(1, args.rows // 10)))
     log(f"generated {rep.n} rows, {len(rep.seq)} letters")
@@ -373,18 +471,19 @@ def main() -> int:
         src.free()
         dst.free()
         log(f"copy calibration: {args.copy_calibration_mib} MiB device-to-device")
-    run = engine.DeviceRun(rep, cl)
+    # Headline: the boundary's own call, rows in host memory at entry and
+    # labels in host memory at exit (H2D copies, clustering, D2H copies).
+    run = engine.HostRun(rep, cl)
     for w in range(args.warmup):
         run()
         log(f"warmup {w + 1}/{args.warmup} done")
     hip.synchronize()
     torch.cuda.synchronize()
-    cnt_ms, fuse_ms, st = [], [], None
+    cnt_ms, st = [], None
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = run()
         cnt_ms.append(st.count_kernel_ms)
-        fuse_ms.append(st.fuse_kernel_ms)
     hip.synchronize()
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
@@ -392,10 +491,15 @@ def main() -> int:
     n = rep.n
     pairs = n * (n - 1) / 2
     s = st.as_dict()
-    log(f"timed {args.steps} steps: {ms:.3f} ms/step; stages {s['stage_ms']}")
+    log(f"timed {args.steps} steps (host memory in and out): {ms:.3f} ms/step; stages {s['stage_ms']}")
+    # The same call with rows already in HBM and labels left there.
+    drun = engine.DeviceRun(rep, cl)
+    dms, _ = timed(drun, args.steps, args.warmup)
+    log(f"device-resident: {dms:.3f} ms/step")
 
     W = max(1, (s["max_len"] + 31) // 32)
     ops = count_kernel_ops(s, rep.vw, rep.jw)
+    fops = filter_ops(s)
     kms = float(np.mean(cnt_ms))
     achieved = ops / (kms * 1e-3) / 1e12
     out = {
@@ -413,13 +517,17 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"exact Levenshtein DBSCAN, one repertoire of {n} aa CDR3s (20k-seed generator), "
-                        "t=2 edits + V/J gate, minPts=3",
+                        "t=2 edits + V/J gate, minPts=3; rows in host memory at entry, labels in host memory "
+                        "at exit",
             "rows": n,
             "mode": "exact",
             "threshold": "abs 2",
             "min_pts": cl.min_pts,
         },
         "seqs_clustered_per_sec": n / (ms * 1e-3),
+        "device_resident": {"ms_per_step": dms, "pairs_per_sec": pairs / (dms * 1e-3),
+                            "seqs_clustered_per_sec": n / (dms * 1e-3),
+                            "note": "rows already in HBM, labels left in HBM"},
         "gated_pairs_per_sec": s["gated_pairs"] / (ms * 1e-3),
         "gated_pairs": s["gated_pairs"],
         "myers_pairs": s["myers_pairs"],
@@ -439,6 +547,9 @@ def main() -> int:
             "traffic": pmc_traffic(n),
             "kernel_ms": kms,
             "algorithmic_ops": ops,
+            "work_model": "gated pairs x (2(Vw+Jw)+2) + Myers text letters x (20W+4(W-1))",
+            "filter_ops": fops,
+            "frac_with_filter_ops": (ops + fops) / (kms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
         },
     }
     # The side sections never take the headline line down with them: a
@@ -450,16 +561,17 @@ def main() -> int:
             log(f"{name} section failed: {exc!r}")
             out[name] = {"error": repr(exc)}
 
+    if not args.no_c1:
+        section("c1", lambda: c1_section(args, threads))
     if not args.no_learned:
-        section("learned", lambda: learned_section(args, cl, rep, ms, run.result()))
+        section("learned", lambda: learned_section(args, cl, rep, dms, drun.result()))
     if not args.no_batched:
-        section("batched", lambda: batched_section(args))
+        section("batched", lambda: batched_section(args, threads))
     if args.c5_rows:
-        section("large", lambda: large_section(args))
+        section("large", lambda: large_section(args, threads))
     if args.gene_rows:
         section("genes", lambda: genes_section(args))
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_rows, n)} rows, {threads} threads")
         out["cpu_baseline"] = cpu_baseline(rep, cl, args.cpu_rows, threads)
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

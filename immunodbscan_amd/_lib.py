@@ -24,6 +24,7 @@ IDB_MEM_DEVICE = 1
 IDB_N_STAGES = 8
 IDB_FLAG_AUTOTUNE = 1
 IDB_FLAG_GRAPH = 2
+IDB_FLAG_FORCE_LARGE_PATHS = 4
 
 STAGE_NAMES = [
     "Identify Duplicates",

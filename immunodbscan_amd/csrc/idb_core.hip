@@ -608,8 +608,11 @@ __device__ __forceinline__ int64_t last_target_block(const QBlock &q, const uint
     return (last - (int64_t)q.s) / IDB_TILE;
 }
 
+// Tiles per query block, as 64-bit values: one bucket of a few million
+// entries (single-bucket mode at 20M rows) makes more than 2^32 tiles, and
+// their total is scanned in 64 bits so that the host sees it whole.
 __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
-                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint32_t *tcnt)
+                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint64_t *tcnt)
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
@@ -620,13 +623,14 @@ __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, con
         }
         const QBlock q = qblock_of(qb, qboff, B, bstart, E);
         const int64_t jhi = last_target_block(q, elen, hilen, banded);
-        tcnt[qb] = jhi >= (int64_t)q.I ? (uint32_t)(jhi - q.I + 1) : 0u;
+        tcnt[qb] = jhi >= (int64_t)q.I ? (uint64_t)(jhi - q.I + 1) : 0u;
     }
 }
 
 // Sizes the host needs, gathered on the device so that one read-back brings
 // them: sz[0..1] = entries (eoff[U], when there are bucket memberships),
-// sz[5] = uniques; later sz[2] = buckets, sz[3] = query blocks, sz[4] = tiles.
+// sz[5] = uniques; later sz[2] = buckets, sz[3] = query blocks, sz[4] and
+// sz[6] = tiles (low and high words).
 __global__ void k_size_entries(const uint64_t *eoff, const uint32_t *num, uint32_t *sz)
 {
     const uint32_t U = num[0];
@@ -636,17 +640,19 @@ __global__ void k_size_entries(const uint64_t *eoff, const uint32_t *num, uint32
     sz[5] = U;
 }
 
-__global__ void k_size_tiles(const uint32_t *num, const uint32_t *qboff, const uint32_t *toff, uint32_t *sz)
+__global__ void k_size_tiles(const uint32_t *num, const uint32_t *qboff, const uint64_t *toff, uint32_t *sz)
 {
     const uint32_t B = num[1], QB = qboff[B];
+    const uint64_t T = toff[QB];
     sz[2] = B;
     sz[3] = QB;
-    sz[4] = toff[QB];
+    sz[4] = (uint32_t)T;
+    sz[6] = (uint32_t)(T >> 32);
 }
 
 __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, const uint64_t *ekey,
-                             uint32_t lb, uint32_t gvj, const uint32_t *toff, idb_tile *tiles)
+                             uint32_t lb, uint32_t gvj, const uint64_t *toff, idb_tile *tiles)
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
@@ -657,7 +663,7 @@ __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, cons
         const uint32_t qn = min((uint32_t)IDB_TILE, q.e - q0);
         // bucket id within its repertoire: the canonical test needs only (v, j)
         const uint32_t bucket = (uint32_t)((ekey[q.s] >> lb) % gvj);
-        uint32_t o = toff[qb];
+        uint64_t o = toff[qb];
         for (int64_t J = q.I; J <= jhi; ++J) {
             const uint32_t t0 = q.s + (uint32_t)J * IDB_TILE;
             const uint32_t tn = min((uint32_t)IDB_TILE, q.e - t0);
@@ -970,6 +976,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
          idb_outputs *o, uint32_t *n_uniques_h, idb_stats *st, const Extra *x = nullptr)
 {
     const bool learned = x && x->enc;
+    const bool force_large = (p->flags & IDB_FLAG_FORCE_LARGE_PATHS) != 0;
     float encoder_ms = 0.f;
     const uint32_t N = d->n_rows;
     const uint32_t vw = d->v_width, jw = d->j_width;
@@ -1114,7 +1121,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     }
     const uint32_t gvj = single ? 1u : (uint32_t)std::min<uint64_t>(gvj64, 0xFFFFFFFFull);
     // threshold tables: count pass, fuse pass, band end
-    std::vector<int16_t> tc(maxlen + 1), tf(maxlen + 1), hl(maxlen + 1);
+    std::array<int16_t, 2049> tc, tf, hl;  // maxlen <= 2048
     for (uint32_t L = 0; L <= maxlen; ++L) {
         const int64_t t = threshold_of(L, *p);
         if (ham) {
@@ -1168,7 +1175,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
         const unsigned ebits = lb + bits_for(max_bucket);
         bool sorted = false;
-        if (E <= IDB_SORT_SMALL_MAX && ebits <= 64) {
+        if (!force_large && E <= IDB_SORT_SMALL_MAX && ebits <= 64) {
             uint32_t *gc = buf<uint32_t>(c, S_RSCNT, (size_t)256 * idb_sort_small_blocks(E));
             const hipError_t e = idb_sort_small(ekey, ekey2, erow, erow2, E, ebits, gc, s);
             if (e == hipSuccess) sorted = true;
@@ -1213,16 +1220,24 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         uint32_t *qboff = buf<uint32_t>(c, S_QBOFF, (size_t)E + 1);
         hipLaunchKernelGGL(k_bucket_blocks, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, bstart, num + 1, E, qbcnt);
         exclusive_scan_total<uint32_t>(c, qbcnt, qboff, (size_t)E + 1);
-        uint32_t *tcount = buf<uint32_t>(c, S_TCNT, (size_t)E + 1);
-        uint32_t *toff = buf<uint32_t>(c, S_TOFF, (size_t)E + 1);
+        uint64_t *tcount = buf<uint64_t>(c, S_TCNT, (size_t)E + 1);
+        uint64_t *toff = buf<uint64_t>(c, S_TOFF, (size_t)E + 1);
         hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, qboff, num + 1, bstart, E,
                            elen, hilen, banded ? 1u : 0u, tcount);
-        exclusive_scan_total<uint32_t>(c, tcount, toff, (size_t)E + 1);
+        exclusive_scan_total<uint64_t>(c, tcount, toff, (size_t)E + 1);
         hipLaunchKernelGGL(k_size_tiles, dim3(1), dim3(64), 0, s, num, qboff, toff, sz);
-        uint32_t bqt[3];
+        uint32_t bqt[5];  // sz[2 .. 6]: buckets, query blocks, tiles (low), uniques, tiles (high)
         readback(c, bqt, sz + 2, sizeof(bqt));
         B = bqt[0];
-        T = bqt[2];
+        const uint64_t T64 = (uint64_t)bqt[2] | ((uint64_t)bqt[4] << 32);
+        // Tile indices are 32-bit in the pair kernels, and a wave's next
+        // index (its current one plus the waves of the grid) must not wrap:
+        // keep 2^24 indices of headroom below 2^32.
+        if (T64 >= (1ull << 32) - (1ull << 24))
+            fail(c, IDB_ERR_UNSUPPORTED, std::to_string(T64) + " tiles in one run: more than " +
+                                             std::to_string((1ull << 32) - (1ull << 24)) +
+                                             " are not supported (split the largest V/J bucket)");
+        T = (uint32_t)T64;
         idb_tile *tiles = buf<idb_tile>(c, S_TILES, std::max<uint32_t>(T, 1u));
         hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(bqt[1])), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
                            banded ? 1u : 0u, ekey2, lb, gvj, toff, tiles);
@@ -1254,7 +1269,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // took the count pass from 3.40 to 3.15 ms).
         edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 64ull * E);
         edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 31);
-        a.chunk_log2 = edge_cap >= (1ull << 24) ? 10u : (edge_cap >= (1ull << 20) ? 8u : 6u);
+        a.chunk_log2 = (force_large || edge_cap >= (1ull << 24)) ? 10u : (edge_cap >= (1ull << 20) ? 8u : 6u);
         a.max_chunks = (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
         edge_cap = (unsigned long long)a.max_chunks << a.chunk_log2;
         a.edges = buf<uint2>(c, S_EDGES, edge_cap);

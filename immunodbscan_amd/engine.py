@@ -80,6 +80,42 @@ def dbscan_batch(rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = Non
     return Result(cid, core, cop, unq, nu[:nr], cnt, st.as_dict())
 
 
+class HostRun:
+    """A repertoire in host memory, clustered the way the reference's
+    boundary does it (InterlinkClass, cuda/Interlink.cu:34-95): each call
+    copies the rows to the device, clusters them and copies clone ids, core
+    flags, copies and uniques back into host arrays.  Buffers are kept across
+    calls."""
+
+    def __init__(self, rep: Repertoire, cl: Clustering, ctx: _lib.Context | None = None, counts: bool = False,
+                 flags: int = 0):
+        self.ctx = ctx or default_context()
+        self.rep = rep
+        n = rep.n
+        self.cid = np.zeros(n, np.uint32)
+        self.core = np.zeros(n, np.int32)
+        self.cop = np.zeros(n, np.uint32)
+        self.unq = np.zeros(n, np.uint32)
+        self.cnt = np.zeros(n, np.uint32) if counts else None
+        # the arrays the dataset points at are kept alive with the run
+        self._keep = (rep.seq if len(rep.seq) else np.zeros(1, np.uint8), np.ascontiguousarray(rep.off),
+                      np.ascontiguousarray(rep.length), np.ascontiguousarray(rep.vbits),
+                      np.ascontiguousarray(rep.jbits))
+        seq, off, ln, vb, jb = self._keep
+        self.ds = _lib.make_dataset(n, seq, off, ln, vb, rep.vw, jb, rep.jw, seq_bytes=len(rep.seq))
+        self.outs = _lib.make_outputs(self.cid, self.core, self.cop, self.unq, self.cnt)
+        self.prm = _params(cl, False, 0, 0, flags)
+        self.stats = _lib.IdbStats()
+
+    def __call__(self) -> _lib.IdbStats:
+        self.ctx.check(_lib.lib.idb_dbscan(self.ctx.handle, self.ds, self.prm, self.outs, self.stats))
+        return self.stats
+
+    def result(self) -> Result:
+        return Result(self.cid, self.core, self.cop, self.unq, int(self.stats.n_uniques), self.cnt,
+                      self.stats.as_dict())
+
+
 class DeviceRun:
     """A repertoire resident in HBM with device-side outputs, for timing the
     path without host copies.  Buffers come from immunodbscan_amd.hip."""

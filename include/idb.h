@@ -76,6 +76,12 @@ typedef struct {
  * union / fuse choice made on the device.  Neither changes any result. */
 #define IDB_FLAG_AUTOTUNE 1u
 #define IDB_FLAG_GRAPH 2u
+/* FORCE_LARGE_PATHS takes, at any size, the code paths that otherwise only
+ * large runs take: the library radix sort of the entry order (used above 2^18
+ * entries) and 1,024-pair chunks of the neighbour-pair list (used for lists
+ * of 2^24 pairs or more).  Results are unchanged; it lets tests check those
+ * paths on inputs that the CPU oracle finishes in seconds. */
+#define IDB_FLAG_FORCE_LARGE_PATHS 4u
 
 /* Per-row outputs (cuda/Interlink.hpp:48-49 and :59-67).  uniques holds the
  * ascending representative indices in its first n_uniques entries and 0 after.

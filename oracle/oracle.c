@@ -606,3 +606,154 @@ int32_t orc_fast_dbscan(const orc_data *d, const orc_params *p, orc_result *r)
     free(rep); free(order); free(copies); free(cnt); free(core); free(ent); free(par); free(bmin); free(root);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Per-row facts for a sample of a large input, without clustering it: R3/R4  */
+/* (representative = smallest equal row, class size) and R8 (the exact        */
+/* copies-weighted count, and the neighbour rows) for each sampled row.  All  */
+/* rows are indexed once by (V gene, J gene, length); a sample then scans the */
+/* length band of its own buckets, and a row z is taken only in the bucket of */
+/* the lowest common V gene and J gene of x and z, so each pair counts once.  */
+/* count(x) = |class(x)| + #{rows z outside class(x) with N(x, z)}, which is   */
+/* copies[x] + sum over unique neighbours y of copies[y] for a representative */
+/* x (every row of y's class has the same predicate as y).  Neighbours are    */
+/* decided by the count pass's predicate with the banded DP (fast_pred; the   */
+/* bucket and the length band already hold the gate and the length test).    */
+static void radix_sort_entries(orc_entry *a, orc_entry *tmp, size_t n, unsigned bits)
+{
+    /* stable LSD passes of 16 bits over the key (rows stay in input order
+     * within equal keys) */
+    size_t *cnt = (size_t *)malloc(sizeof(size_t) * 65536);
+    for (unsigned sh = 0; sh < bits; sh += 16) {
+        memset(cnt, 0, sizeof(size_t) * 65536);
+        for (size_t i = 0; i < n; ++i) cnt[(a[i].key >> sh) & 0xFFFFu]++;
+        size_t s = 0;
+        for (size_t b = 0; b < 65536; ++b) { size_t c = cnt[b]; cnt[b] = s; s += c; }
+        for (size_t i = 0; i < n; ++i) tmp[cnt[(a[i].key >> sh) & 0xFFFFu]++] = a[i];
+        memcpy(a, tmp, sizeof(orc_entry) * n);
+    }
+    free(cnt);
+}
+
+static size_t lower_key(const orc_entry *e, size_t n, uint64_t key)
+{
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        if (e[mid].key < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+typedef struct { uint32_t *v; size_t n, cap; } orc_u32vec;
+
+static void u32vec_push(orc_u32vec *v, uint32_t x)
+{
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 64;
+        v->v = (uint32_t *)realloc(v->v, sizeof(uint32_t) * v->cap);
+    }
+    v->v[v->n++] = x;
+}
+
+int32_t orc_sample_rows(const orc_data *d, const orc_params *p, const uint32_t *xs, uint32_t n_x, uint32_t *rep,
+                        uint32_t *copies, uint32_t *count, uint64_t *nbr_off, uint32_t *nbr, uint64_t nbr_cap)
+{
+    const uint32_t n = d->n;
+    for (uint32_t k = 0; k < n_x; ++k)
+        if (xs[k] >= n) return -1;
+    const uint64_t gj = 32ull * d->jw;
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < n; ++i) if (d->len[i] > maxlen) maxlen = d->len[i];
+    size_t ne = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t pv = 0, pj = 0;
+        for (uint32_t w = 0; w < d->vw; ++w) pv += __builtin_popcount((uint32_t)d->vbits[(size_t)i * d->vw + w]);
+        for (uint32_t w = 0; w < d->jw; ++w) pj += __builtin_popcount((uint32_t)d->jbits[(size_t)i * d->jw + w]);
+        ne += (size_t)pv * pj;
+    }
+    orc_entry *ent = (orc_entry *)malloc(sizeof(orc_entry) * (ne ? ne : 1));
+    orc_entry *tmp = (orc_entry *)malloc(sizeof(orc_entry) * (ne ? ne : 1));
+    size_t k = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t v = 0; v < 32 * d->vw; ++v) {
+            if (!((uint32_t)d->vbits[(size_t)i * d->vw + (v >> 5)] >> (v & 31) & 1u)) continue;
+            for (uint32_t j = 0; j < 32 * d->jw; ++j) {
+                if (!((uint32_t)d->jbits[(size_t)i * d->jw + (j >> 5)] >> (j & 31) & 1u)) continue;
+                ent[k].key = (((uint64_t)v * gj + j) << 16) | d->len[i];
+                ent[k].row = i;
+                ++k;
+            }
+        }
+    unsigned bits = 16;
+    while (bits < 64 && ((32ull * d->vw * gj) >> (bits - 16))) bits += 16;
+    radix_sort_entries(ent, tmp, ne, bits);
+    free(tmp);
+
+    orc_u32vec *nb = (orc_u32vec *)calloc(n_x ? n_x : 1, sizeof(orc_u32vec));
+    const int nt = nthreads_of(p);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int64_t s = 0; s < (int64_t)n_x; ++s) {
+        const uint32_t x = xs[s], lx = d->len[x];
+        const int32_t *vx = d->vbits + (size_t)x * d->vw, *jx = d->jbits + (size_t)x * d->jw;
+        const int vlow = lowest_common_bit(vx, vx, d->vw), jlow = lowest_common_bit(jx, jx, d->jw);
+        uint32_t r = x, c = 0;
+        /* R3 class of x: equal rows share every bucket of x (and rows with no
+         * V or no J gene have no bucket: a linear scan finds theirs) */
+        if (vlow < 0 || jlow < 0) {
+            for (uint32_t z = 0; z < n; ++z)
+                if (is_duplicate(d, x, z)) { ++c; if (z < r) r = z; }
+        } else {
+            const uint64_t key = (((uint64_t)vlow * gj + (uint64_t)jlow) << 16) | lx;
+            for (size_t e = lower_key(ent, ne, key); e < ne && ent[e].key == key; ++e)
+                if (is_duplicate(d, x, ent[e].row)) { ++c; if (ent[e].row < r) r = ent[e].row; }
+        }
+        rep[s] = r;
+        copies[s] = c;
+        uint32_t cntx = c;
+        if (vlow >= 0 && jlow >= 0) {
+            for (uint32_t v = (uint32_t)vlow; v < 32 * d->vw; ++v) {
+                if (!((uint32_t)vx[v >> 5] >> (v & 31) & 1u)) continue;
+                for (uint32_t j = (uint32_t)jlow; j < 32 * d->jw; ++j) {
+                    if (!((uint32_t)jx[j >> 5] >> (j & 31) & 1u)) continue;
+                    const uint64_t bkt = (uint64_t)v * gj + j;
+                    for (uint32_t L = 0; L <= maxlen; ++L) {
+                        /* the lengths a neighbour can have: equal (Hamming)
+                         * or |lx - L| <= t(lx, L) (Levenshtein) */
+                        if (p->use_hamming) {
+                            if (L != lx) continue;
+                        } else {
+                            int64_t dl = (int64_t)lx - (int64_t)L;
+                            if (dl < 0) dl = -dl;
+                            if (dl > orc_threshold(lx, L, p->eps, p->abs_k)) continue;
+                        }
+                        const uint64_t key = (bkt << 16) | L;
+                        for (size_t e = lower_key(ent, ne, key); e < ne && ent[e].key == key; ++e) {
+                            const uint32_t z = ent[e].row;
+                            if (z == x || is_duplicate(d, x, z)) continue;
+                            const int32_t *vz = d->vbits + (size_t)z * d->vw, *jz = d->jbits + (size_t)z * d->jw;
+                            if (lowest_common_bit(vx, vz, d->vw) != (int)v || lowest_common_bit(jx, jz, d->jw) != (int)j)
+                                continue;
+                            int fuse_ok = 0;  /* the count pass's predicate, banded */
+                            if (!fast_pred(d, p, x, z, &fuse_ok)) continue;
+                            ++cntx;
+                            u32vec_push(&nb[s], z);
+                        }
+                    }
+                }
+            }
+        }
+        count[s] = cntx;
+    }
+    uint64_t o = 0;
+    for (uint32_t s = 0; s < n_x; ++s) {
+        nbr_off[s] = o;
+        for (size_t m = 0; m < nb[s].n; ++m, ++o)
+            if (o < nbr_cap) nbr[o] = nb[s].v[m];
+        free(nb[s].v);
+    }
+    nbr_off[n_x] = o;
+    free(nb);
+    free(ent);
+    return 0;
+}

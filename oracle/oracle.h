@@ -70,6 +70,15 @@ int32_t orc_neighbour_count_pass(const orc_data *d, const orc_params *p, uint32_
 int32_t orc_neighbour_fuse_pass(const orc_data *d, const orc_params *p, uint32_t x, uint32_t y);
 
 int32_t orc_ref_dbscan(const orc_data *d, const orc_params *p, orc_result *r);
+/* Facts for the sampled rows xs[0 .. n_x) of an input too large to cluster
+ * on the CPU: rep[s] = smallest row equal to xs[s] (R3/R4), copies[s] = its
+ * class size, count[s] = |class| + rows outside the class that are count-pass
+ * neighbours (the exact R8 count when xs[s] is a representative), and those
+ * neighbour rows in nbr[nbr_off[s] .. nbr_off[s+1]) (nbr_off has n_x + 1
+ * entries; rows past nbr_cap are not written, nbr_off[n_x] is the total).
+ * Returns -1 for a sample row out of range. */
+int32_t orc_sample_rows(const orc_data *d, const orc_params *p, const uint32_t *xs, uint32_t n_x, uint32_t *rep,
+                        uint32_t *copies, uint32_t *count, uint64_t *nbr_off, uint32_t *nbr, uint64_t nbr_cap);
 int32_t orc_fast_dbscan(const orc_data *d, const orc_params *p, orc_result *r);
 
 #ifdef __cplusplus

@@ -64,6 +64,9 @@ def lib():
         for f in ("orc_ref_dbscan", "orc_fast_dbscan"):
             getattr(L, f).argtypes = [C.POINTER(OrcData), C.POINTER(OrcParams), C.POINTER(OrcResult)]
             getattr(L, f).restype = C.c_int32
+        L.orc_sample_rows.argtypes = [C.POINTER(OrcData), C.POINTER(OrcParams), vp, C.c_uint32, vp, vp, vp, vp, vp,
+                                      C.c_uint64]
+        L.orc_sample_rows.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -154,3 +157,22 @@ def dbscan(data: Data, prm: OrcParams, fast: bool = False) -> dict:
 def neighbour(data: Data, prm: OrcParams, x: int, y: int, fuse: bool = False) -> bool:
     f = lib().orc_neighbour_fuse_pass if fuse else lib().orc_neighbour_count_pass
     return bool(f(C.byref(data.c), C.byref(prm), x, y))
+
+
+def sample_rows(data: Data, prm: OrcParams, rows, nbr_cap: int = 1 << 26) -> dict:
+    """Representative, class size, exact count and count-pass neighbour rows
+    of each sampled row (orc_sample_rows), for inputs too large to cluster."""
+    xs = np.ascontiguousarray(rows, np.uint32)
+    m = len(xs)
+    rep, cop, cnt = (np.zeros(m, np.uint32) for _ in range(3))
+    off = np.zeros(m + 1, np.uint64)
+    nbr = np.zeros(max(nbr_cap, 1), np.uint32)
+    rc = lib().orc_sample_rows(C.byref(data.c), C.byref(prm), xs.ctypes.data, m, rep.ctypes.data, cop.ctypes.data,
+                               cnt.ctypes.data, off.ctypes.data, nbr.ctypes.data, nbr_cap)
+    if rc != 0:
+        raise RuntimeError(f"oracle sample failed with {rc}")
+    total = int(off[m])
+    if total > nbr_cap:
+        raise RuntimeError(f"{total} neighbour rows exceed the buffer of {nbr_cap}")
+    return {"rep": rep, "copies": cop, "count": cnt,
+            "neighbours": [nbr[int(off[s]):int(off[s + 1])] for s in range(m)]}

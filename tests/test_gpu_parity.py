@@ -297,3 +297,27 @@ def test_interlink_cpp_drop_in(gpu_ctx, tmp_path):
         assert np.array_equal(got[:, 1], exp["copies"])
         assert np.array_equal(got[:, 2], exp["core"])
         assert f"# n_uniques {exp['n_uniques']}" in out.stdout
+
+
+def test_large_input_paths_forced_on_small_inputs(gpu_ctx):
+    """The library radix sort of the entry order (otherwise above 2^18
+    entries) and 1,024-pair chunks (otherwise for pair lists of 2^24 or more),
+    switched on for inputs the oracle checks in seconds."""
+    F = _lib.IDB_FLAG_FORCE_LARGE_PATHS
+    rep = synth.generate(configs.c2_params(n_rows=20_000, n_seeds=2_000))
+    check(rep, configs.C2, fast=True, tag="forced c2", flags=F)
+    check(synth.generate(configs.c1_params()), configs.C1, tag="forced c1", flags=F)
+    check(synth.generate(configs.c5_params(n_rows=20_000, n_seeds=200)), configs.C5, fast=True, tag="forced c5",
+          flags=F)
+    # a capacity of four 1,024-pair chunks: the waves overflow it and the
+    # fuse pass grows the clusters
+    got, _ = check(rep, configs.C2, fast=True, tag="forced c2 overflow", flags=F, edge_capacity=4096)
+    assert got.stats["fuse_from_edges"] == 0
+    reps = configs.c3_repertoires(n_reps=24, scale=0.05)
+    cat = synth.concat(reps)
+    got = engine.dbscan_batch(cat, configs.C3, flags=F | _lib.IDB_FLAG_GRAPH)
+    for r, rp in enumerate(reps):
+        s, e = int(cat.rep_start[r]), int(cat.rep_start[r + 1])
+        exp = oracle_of(rp, configs.C3)
+        for k in ("clone_id", "core", "copies", "uniques", "counts"):
+            assert np.array_equal(getattr(got, k)[s:e], exp[k]), (r, k)

@@ -491,3 +491,29 @@ def test_invariants_on_c1():
     # counts: sum over uniques of (count - copies) equals, over neighbour pairs,
     # copies[x] + copies[y]; both sides are even only if ... check parity via edges
     assert np.all(r["counts"][u] >= r["copies"][u])
+
+
+# --------------------------------------------------------------------------
+# Sampled facts of large inputs (orc_sample_rows): against brute force over
+# all pairs and the literal all-pairs oracle's counts.
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_sample_rows_equals_brute_force(cfg):
+    rng = random.Random(500 + CONFIGS.index(cfg))
+    rows = random_rows(rng, 300, b"ACGT", 5, 12, 40, 3, 0.4, p_empty=0.05, families=30, edits=2)
+    data, seqs, vb, jb = make_rows(rows)
+    prm = O.params(**cfg)
+    ref = O.dbscan(data, prm, fast=False)
+    n = len(rows)
+    key = [(seqs[i], vb[i].tobytes(), jb[i].tobytes()) for i in range(n)]
+    first = {}
+    for i, k in enumerate(key):
+        first.setdefault(k, i)
+    rep = np.array([first[k] for k in key], np.uint32)
+    size = np.bincount(rep, minlength=n)
+    xs = rng.sample(range(n), 120)
+    got = O.sample_rows(data, prm, xs)
+    for s, x in enumerate(xs):
+        assert got["rep"][s] == rep[x] and got["copies"][s] == size[rep[x]], x
+        assert got["count"][s] == ref["counts"][rep[x]], (x, got["count"][s], ref["counts"][rep[x]])
+        nb = sorted(z for z in range(n) if rep[z] != rep[x] and O.neighbour(data, prm, x, z))
+        assert sorted(got["neighbours"][s].tolist()) == nb, x
