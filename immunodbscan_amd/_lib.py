@@ -25,6 +25,7 @@ IDB_N_STAGES = 8
 IDB_FLAG_AUTOTUNE = 1
 IDB_FLAG_GRAPH = 2
 IDB_FLAG_FORCE_LARGE_PATHS = 4
+IDB_NOALLOC_MAX_ROWS = 65536
 
 STAGE_NAMES = [
     "Identify Duplicates",
@@ -44,6 +45,7 @@ EXPORTED = [
     "idb_gene_bitsets",
     "idb_gene_bitsets_gpu",
     "idb_dbscan",
+    "idb_dbscan_noalloc",
     "idb_dbscan_batch",
     "idb_pair_distances",
     "idb_neighbour_pairs",
@@ -184,6 +186,9 @@ def _load() -> C.CDLL:
     lib.idb_dbscan.argtypes = [vp, C.POINTER(IdbDataset), C.POINTER(IdbParams), C.POINTER(IdbOutputs),
                                C.POINTER(IdbStats)]
     lib.idb_dbscan.restype = C.c_int
+    lib.idb_dbscan_noalloc.argtypes = [vp, C.POINTER(IdbDataset), C.POINTER(IdbParams), C.POINTER(IdbOutputs),
+                                       C.POINTER(IdbStats)]
+    lib.idb_dbscan_noalloc.restype = C.c_int
     lib.idb_dbscan_batch.argtypes = [vp, C.POINTER(IdbDataset), vp, u32, C.POINTER(IdbParams),
                                      C.POINTER(IdbOutputs), vp, C.POINTER(IdbStats)]
     lib.idb_dbscan_batch.restype = C.c_int

@@ -28,10 +28,38 @@
 #include <string_view>
 #include <array>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/idb.h"
 #include "idb_internal.h"
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free runs: one static device array, which every
+// GPU holds at its own address for the life of the process.  Host code never
+// uses its name as an address (in host code it is only a stand-in): the
+// device address is looked up once per device (scratch_on) and kept.
+static __device__ __align__(256) uint8_t scratch[IDB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, uint8_t *> g_scratch_at;  // device -> address of its scratch array
+
+// The device address of `device`'s scratch array; the device must be current.
+hipError_t scratch_on(int device, uint8_t **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_at.find(device);
+    if (it == g_scratch_at.end()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+        it = g_scratch_at.emplace(device, (uint8_t *)p).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Context and device arena
@@ -61,6 +89,13 @@ struct idb_ctx {
     };
     std::map<std::array<uint32_t, 6>, Tuned> tuned;
     hipGraphExec_t gexec = nullptr;  // the tail graph, updated in place
+    // Allocation-free runs (idb_dbscan_noalloc): every buffer of the call is
+    // carved from this device's scratch array instead of the arena.
+    uint8_t *scratch = nullptr;   // device address of the scratch array on this device
+    bool noalloc = false;         // the current call takes buffers from scratch
+    size_t na_used = 0;           // bytes handed out in this call
+    void *na_ptr[S_COUNT] = {};
+    size_t na_cap[S_COUNT] = {};
 };
 
 namespace {
@@ -88,6 +123,20 @@ T *buf(idb_ctx *c, Slot s, size_t count)
 {
     size_t bytes = count * sizeof(T);
     if (bytes == 0) bytes = 256;
+    if (c->noalloc) {
+        // bump allocation in the scratch array, one region per slot and call
+        if (c->na_cap[s] >= bytes) return (T *)c->na_ptr[s];
+        const size_t at = (c->na_used + 255) & ~(size_t)255;
+        if (at + bytes > IDB_NOALLOC_SCRATCH_BYTES)
+            fail(c, IDB_ERR_UNSUPPORTED, "the allocation-free run needs more than its " +
+                                             std::to_string(IDB_NOALLOC_SCRATCH_BYTES >> 20) +
+                                             " MiB scratch for this input (" + std::to_string(at + bytes) +
+                                             " bytes at least)");
+        c->na_ptr[s] = c->scratch + at;
+        c->na_cap[s] = bytes;
+        c->na_used = at + bytes;
+        return (T *)c->na_ptr[s];
+    }
     if (c->cap[s] < bytes) {
         if (c->ptr[s]) (void)hipFree(c->ptr[s]);
         c->ptr[s] = nullptr;
@@ -1269,8 +1318,22 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // took the count pass from 3.40 to 3.15 ms).
         edge_cap = p->edge_capacity ? p->edge_capacity : std::max<unsigned long long>(1ull << 22, 64ull * E);
         edge_cap = std::min<unsigned long long>(edge_cap, 1ull << 31);
+        if (c->noalloc) {
+            // Allocation-free run: the pair list gets what the scratch holds
+            // after the buffers still to come (core flags, labels, output
+            // staging: 28 bytes per row, plus 1 MiB for the scans and
+            // alignment), at 12 bytes per pair (the pair and its chunk's
+            // share of the fill counts); a smaller list only sends cluster
+            // growth to the fuse pass.
+            const size_t used = (c->na_used + 255) & ~(size_t)255;
+            const size_t later = 28ull * N + (1ull << 20);
+            const size_t room = IDB_NOALLOC_SCRATCH_BYTES > used + later ? IDB_NOALLOC_SCRATCH_BYTES - used - later : 0;
+            edge_cap = std::min<unsigned long long>(edge_cap, room / 12);
+            if (edge_cap < IDB_TILE) fail(c, IDB_ERR_UNSUPPORTED, "the allocation-free run's scratch is too small for this input");
+        }
         a.chunk_log2 = (force_large || edge_cap >= (1ull << 24)) ? 10u : (edge_cap >= (1ull << 20) ? 8u : 6u);
-        a.max_chunks = (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
+        a.max_chunks = c->noalloc ? (uint32_t)std::max<unsigned long long>(1ull, edge_cap >> a.chunk_log2)
+                                  : (uint32_t)((edge_cap + (1ull << a.chunk_log2) - 1) >> a.chunk_log2);
         edge_cap = (unsigned long long)a.max_chunks << a.chunk_log2;
         a.edges = buf<uint2>(c, S_EDGES, edge_cap);
         a.chunk_fill = buf<uint32_t>(c, S_FILL, a.max_chunks);
@@ -1539,6 +1602,12 @@ int idb_create(idb_ctx **out, int device)
             delete c;
             return IDB_ERR_HIP;
         }
+    // the scratch array's address on this device (the device is current)
+    if (scratch_on(device, &c->scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        idb_destroy(c);
+        return IDB_ERR_HIP;
+    }
     *out = c;
     return IDB_OK;
 }
@@ -1586,7 +1655,7 @@ int idb_gene_bitsets(uint32_t n, const uint8_t *calls, const uint64_t *call_off,
 }
 
 static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uint32_t nr, const idb_params *p,
-                       idb_outputs *o, uint32_t *nu, idb_stats *st, const Extra *x = nullptr)
+                       idb_outputs *o, uint32_t *nu, idb_stats *st, const Extra *x = nullptr, bool noalloc = false)
 {
     if (!c) return IDB_ERR_ARG;
     if (!d || !p || !o) {
@@ -1594,6 +1663,20 @@ static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uin
         return IDB_ERR_ARG;
     }
     c->err.clear();
+    // allocation-free mode lasts for this call only
+    struct Mode {
+        idb_ctx *c;
+        Mode(idb_ctx *cc, bool on) : c(cc)
+        {
+            c->noalloc = on;
+            c->na_used = 0;
+            for (int k = 0; k < S_COUNT; ++k) {
+                c->na_ptr[k] = nullptr;
+                c->na_cap[k] = 0;
+            }
+        }
+        ~Mode() { c->noalloc = false; }
+    } mode(c, noalloc);
     try {
         check(c, hipSetDevice(c->device), "hipSetDevice");
         run(c, d, rs, nr, p, o, nu, st, x);
@@ -1611,6 +1694,29 @@ static int run_guarded(idb_ctx *c, const idb_dataset *d, const uint32_t *rs, uin
 int idb_dbscan(idb_ctx *c, const idb_dataset *d, const idb_params *p, idb_outputs *o, idb_stats *st)
 {
     return run_guarded(c, d, nullptr, 0, p, o, nullptr, st);
+}
+
+int idb_dbscan_noalloc(idb_ctx *c, const idb_dataset *d, const idb_params *p, idb_outputs *o, idb_stats *st)
+{
+    if (!c) return IDB_ERR_ARG;
+    if (!d || !p || !o) {
+        c->err = "null argument";
+        return IDB_ERR_ARG;
+    }
+    if (p->flags != 0) {
+        c->err = "the allocation-free run takes no flags (autotuning and graph capture allocate)";
+        return IDB_ERR_ARG;
+    }
+    if (d->n_rows > IDB_NOALLOC_MAX_ROWS) {
+        c->err = "the allocation-free run takes at most " + std::to_string(IDB_NOALLOC_MAX_ROWS) + " rows, got " +
+                 std::to_string(d->n_rows);
+        return IDB_ERR_UNSUPPORTED;
+    }
+    if (!c->scratch) {
+        c->err = "no scratch array on this device";
+        return IDB_ERR_HIP;
+    }
+    return run_guarded(c, d, nullptr, 0, p, o, nullptr, st, nullptr, true);
 }
 
 int idb_dbscan_batch(idb_ctx *c, const idb_dataset *d, const uint32_t *rep_start, uint32_t n_reps, const idb_params *p,

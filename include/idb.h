@@ -155,6 +155,26 @@ int idb_gene_bitsets_gpu(idb_ctx *ctx, uint32_t mem, uint32_t n, const uint8_t *
 int idb_dbscan(idb_ctx *ctx, const idb_dataset *data, const idb_params *params, idb_outputs *out,
                idb_stats *stats);
 
+/* Allocation-free variant of idb_dbscan, with the same arguments and the
+ * same results.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the run (input and output staging,
+ * duplicate table, bucket entries, tiles, neighbour-pair list, scan space)
+ * comes from one static device array of IDB_NOALLOC_SCRATCH_BYTES per GPU,
+ * which the library holds until the process ends; the call itself makes no
+ * hipMalloc, hipFree or other allocating call.  Limits: at most
+ * IDB_NOALLOC_MAX_ROWS rows (IDB_ERR_UNSUPPORTED above), and inputs whose
+ * entries and tiles do not fit in the scratch fail with IDB_ERR_UNSUPPORTED;
+ * the neighbour-pair list takes what is left (a shorter list only sends
+ * cluster growth to the second pass over the tiles).  params->flags must be
+ * 0 (autotuning and graph capture allocate).  Every call on one GPU shares
+ * that GPU's scratch array: calls that overlap on one GPU (two contexts on
+ * two host threads) race and must be serialised by the caller. */
+#define IDB_NOALLOC_MAX_ROWS 65536u
+#define IDB_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int idb_dbscan_noalloc(idb_ctx *ctx, const idb_dataset *data, const idb_params *params, idb_outputs *out,
+                       idb_stats *stats);
+
 /* Many repertoires in one pass: data holds the rows of all repertoires back to
  * back, repertoire r being rows rep_start[r] .. rep_start[r+1]-1.  Results
  * equal n_reps separate idb_dbscan calls: clone ids restart at 1, and uniques
