@@ -49,7 +49,8 @@ def build_hip() -> str:
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         list(ex.map(_run, jobs))
     out = os.path.join(PKG, "libidb_hip.so")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
+          "-Wl,-rpath,/opt/rocm/lib", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

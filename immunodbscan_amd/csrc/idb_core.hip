@@ -31,6 +31,8 @@
 #include <mutex>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/idb.h"
 #include "idb_internal.h"
 
@@ -790,9 +792,16 @@ __global__ void k_roots(const uint32_t *uniq, const uint32_t *num, const int32_t
     }
 }
 
+// Depth histogram over the row forest (the reference's
+// tree_depth_evaluation_kernel, cuda/PSDBSCAN.hcu:154-177) for runs whose
+// clusters grew in row space: the fuse pass after a pair-list overflow, or
+// no bucket entries at all.  Runs grown from the pair list get theirs from
+// the entry forest (k_union_depth) and skip this one (only_if_overflow).
 __global__ void k_tree_depth(const uint32_t *rep, const int32_t *core, const uint32_t *par, const uint32_t *bmin,
-                             uint32_t n, uint32_t *hist)
+                             uint32_t n, uint32_t *hist, const uint32_t *n_chunks, uint32_t max_chunks,
+                             uint32_t only_if_overflow)
 {
+    if (only_if_overflow && *n_chunks <= max_chunks) return;
     // a block histogram in LDS, added to the run's histogram once per bin
     __shared__ uint32_t h[51];
     for (uint32_t k = threadIdx.x; k < 51; k += blockDim.x) h[k] = 0u;
@@ -942,6 +951,27 @@ unsigned bits_for(uint64_t v)
 
 void record(idb_ctx *c, int k) { check(c, hipEventRecord(c->ev[k], c->stream), "event record"); }
 
+// roctx ranges named after the reference's stage timers
+// (cuda/PSDBSCAN.hcu:1301-1354): one outer range for the whole run and one
+// for the stage being enqueued; the destructor closes what is open, also
+// when a run fails.
+struct StageRanges {
+    int depth = 0;
+    StageRanges() { depth += roctxRangePushA("Clustering core process") >= 0; }
+    void stage(const char *name)
+    {
+        if (depth > 1) {
+            roctxRangePop();
+            --depth;
+        }
+        depth += roctxRangePushA(name) >= 0;
+    }
+    ~StageRanges()
+    {
+        while (depth-- > 0) roctxRangePop();
+    }
+};
+
 // Autotuner of the count kernel's launch shape: waves per block (1, 2, 4 as
 // LDS allows), grid of 1, 2, 4 or 8 times the resident blocks, XCD-grouped or
 // plain block order.  Each candidate runs the real count pass on the real
@@ -1047,6 +1077,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     }
     hipStream_t s = c->stream;
     record(c, 0);
+    StageRanges rng;
+    rng.stage("Identify Duplicates");
 
     // inputs
     const uint8_t *seq = stage_in<uint8_t>(c, S_SEQ, d->seq, d->seq_bytes, d->mem);
@@ -1092,12 +1124,14 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     hipLaunchKernelGGL(k_dedup_resolve, dim3(blocks_for(N)), dim3(256), 0, s, htab, gslot, N, rep, copies, flag, cnt, par,
                        bmin, rflag, pe);
     record(c, 1);
+    rng.stage("Determine copies");
 
     // --- Determine copies: uniques
     uint32_t *uniq = buf<uint32_t>(c, S_UNIQ, N);
     uint32_t *num = buf<uint32_t>(c, S_NUM, 4);
     select_flagged(c, N, flag, uniq, num);
     record(c, 2);
+    rng.stage("Identify core points");
 
     // --- Identify core points: buckets, tiles, count pass
     const uint32_t gj = 32 * std::max(jw, 1u);
@@ -1260,6 +1294,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         ua.n_rows = N;
         ua.par = par;
         ua.bmin = bmin;
+        ua.rep = rep;
+        ua.depth_hist = p->tree_stats ? small + 16 : nullptr;
         uint32_t *bstart = buf<uint32_t>(c, S_BSTART, E);
         select_flagged(c, E, bflag, bstart, num + 1);
         // Buckets B <= E and query blocks QB <= E stay on the device: the
@@ -1306,6 +1342,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.gj = gj;
         a.single_bucket = single;
         a.band_filter = filt ? 1u : 0u;
+        // the absolute mode gives every pair the same count-pass threshold
+        a.t_const = (p->abs_k >= 0 && !ham) ? (int32_t)tc[0] : -1;
         a.qgram_filter = qgram ? 1u : 0u;
         a.e_qg = eqg;
         a.cnt = cnt;
@@ -1378,6 +1416,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, pe,
                            small);
         if (!g) record(c, 3);
+        rng.stage("Grow clusters");
 
         // --- Grow clusters: union over the recorded neighbour pairs, or a
         // fuse pass over the tiles when the pair list overflowed
@@ -1435,12 +1474,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             }
         }
         if (!g) record(c, 4);
+        rng.stage("Reduction of cluster information tree structure");
 
         // --- Reduction of cluster information tree structure
         hipLaunchKernelGGL(k_roots, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, core, par, bmin, lab, rflag, N);
         if (p->tree_stats)
-            hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, small + 16);
+            hipLaunchKernelGGL(k_tree_depth, dim3(blocks_for(N)), dim3(256), 0, s, rep, core, par, bmin, N, small + 16,
+                               small + 12, a.max_chunks, E > 0 ? 1u : 0u);
         if (!g) record(c, 5);
+        rng.stage("Re-number clusters and identify noise");
 
         // --- Re-number clusters and identify noise
         exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);

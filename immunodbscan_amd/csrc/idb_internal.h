@@ -123,6 +123,7 @@ struct idb_pair_args {
     uint32_t max_chunks;          // capacity in chunks
     uint32_t chunk_log2;          // pairs per chunk = 1 << chunk_log2 (>= IDB_TILE)
     uint32_t fuse_if_overflow;    // fuse pass: run only when the pair list overflowed
+    int32_t t_const;              // >= 0: every pair's count-pass threshold (absolute mode); -1 otherwise
     const uint16_t *zb;           // learned filter: bfloat16 embedding (64) per row
     float min_sim;                // learned filter: proposal threshold on z_x . z_y
     // launch shape (0 = default): waves per block, grid = resident blocks x
@@ -162,6 +163,8 @@ struct idb_union_args {
     uint32_t n_entries, n_rows;
     uint32_t *par;                // per row: set to the component minimum for core rows
     uint32_t *bmin;               // per row: border claims
+    const uint32_t *rep;          // per row: representative (for the depth histogram)
+    uint32_t *depth_hist;         // 51 bins of union-find depths, or null (no histogram)
 };
 hipError_t idb_launch_union_edges(const idb_pair_args &a, const idb_union_args &u, hipStream_t s);
 

@@ -352,6 +352,144 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
 }
 
+// Myers restricted to a diagonal band (patterns of 3 or more words, pair
+// thresholds of at most 15).  Only lev(x, y) <= t is asked, and E. Ukkonen
+// ("Algorithms for approximate string matching", Inf. Control 1985) showed
+// that cells with |i - j| > t cannot lie on a path of cost <= t; so at text
+// letter j only the pattern rows j - K .. j + K matter (K = the largest
+// threshold of the batch, uniform over the wave), which lie in two
+// consecutive 32-row words.  Each step runs the usual multi-word recurrence
+// on that two-word window only:
+//   * the words below the window were never touched and still hold the
+//     first column's vertical deltas (all +1), which is a valid assignment
+//     of values larger than K to cells outside the band; when the window
+//     moves down a word, the new lower word starts from that state;
+//   * the word above the window is out of the band, and the boundary row
+//     above the window gets horizontal delta +1 per letter (shift-in 1, add
+//     carry 0), as row 0 does in the unbanded recurrence;
+//   * the tracked score is D[B][j] + (m - B) for the window's last row
+//     B = min(m, 32 (lo + 2)) (rows below B count +1 each), which moves by
+//     the horizontal delta at row B, and equals D[m][n] at the end because
+//     row m is inside the band at the last letter (|m - n| <= t).
+// Every value computed this way is at least min(true value, K + 1), and a
+// cell whose true value is at most K is exact (its optimal path stays in the
+// band), so dist <= t decides exactly as the unbanded recurrence does.
+template <int W, bool FUSE>
+__device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
+                                                 const uint16_t *queue, uint32_t n, uint32_t lane,
+                                                 const idb_entry &qe, const idb_entry &te, int32_t core_y,
+                                                 int32_t q_core, EdgeStage &est, int32_t *qacc, int32_t *tacc,
+                                                 unsigned long long &hits, unsigned long long &mlet, uint32_t K)
+{
+    const bool act = lane < n;
+    const uint32_t e = queue[lane < n ? lane : 0u];
+    const int qi = (int)(e >> 8), tl = (int)(e & 0xFFu);
+    const uint32_t ty = (uint32_t)__shfl((int)te.row, tl, 64);
+    const uint32_t tm = (uint32_t)__shfl((int)te.len_chk, tl, 64) & 0xFFFFu;
+    const uint32_t tcy = (uint32_t)__shfl((int)te.cp, tl, 64);
+    const uint32_t tt2 = (uint32_t)__shfl((int)te.t2, tl, 64);
+    const int32_t tcore = __shfl(core_y, tl, 64);
+    const uint32_t x = (uint32_t)__shfl((int)qe.row, qi, 64);
+    const uint32_t lxq = (uint32_t)__shfl((int)qe.len_chk, qi, 64) & 0xFFFFu;
+    const uint32_t cx = (uint32_t)__shfl((int)qe.cp, qi, 64);
+    const uint32_t qt2 = (uint32_t)__shfl((int)qe.t2, qi, 64);
+    const int32_t core_x = __shfl(q_core, qi, 64);
+    const uint32_t lx = act ? lxq : 0u;
+    const int32_t t = !act ? -1 : (lx >= tm ? t_of(qt2, FUSE) : t_of(tt2, FUSE));
+    mlet += lx;
+    // two-word window [lo, lo + 1] over the pattern rows (lo <= W - 2)
+    uint32_t pa = 0xFFFFFFFFu, ma = 0u, pb = 0xFFFFFFFFu, mb = 0u;
+    int32_t score = (int32_t)tm;
+    uint32_t lo = 0;
+    const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
+    const uint32_t *pcol = peq + tl;
+    uint32_t lmax = lx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, o, 64));
+    lmax = __builtin_amdgcn_readfirstlane(lmax);
+    for (uint32_t j0 = 0; j0 < lmax; j0 += 4) {
+        const uint32_t word = j0 < lx ? xc[j0 >> 2] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t j = j0 + k + 1;  // 1-based text column
+            if (j > lmax) break;
+            // window of column j: the word of row j - K, kept within [0, W - 2]
+            const uint32_t want = j > K + 1 ? min((j - K - 1) >> 5, (uint32_t)W - 2u) : 0u;
+            if (want != lo) {  // uniform: one word further down
+                pa = pb;
+                ma = mb;
+                pb = 0xFFFFFFFFu;
+                mb = 0u;
+                lo = want;
+            }
+            const uint32_t c = (word >> (8 * k)) & 0xFFu;
+            const uint32_t ea = pcol[(c * W + lo) * 64], eb = pcol[(c * W + lo + 1) * 64];
+            // word a: boundary row above it (shift-in +1, no add carry)
+            const uint32_t xva = ea | ma;
+            const uint64_t sa64 = (uint64_t)(ea & pa) + pa;
+            const uint32_t xha = ((uint32_t)sa64 ^ pa) | ea;
+            const uint32_t pha = ma | ~(xha | pa);
+            const uint32_t mha = pa & xha;
+            // word b: carries from word a
+            const uint32_t cin = (uint32_t)(sa64 >> 32);
+            const uint32_t xvb = eb | mb;
+            const uint32_t sb = (eb & pb) + pb + cin;
+            const uint32_t xhb = (sb ^ pb) | eb;
+            const uint32_t phb = mb | ~(xhb | pb);
+            const uint32_t mhb = pb & xhb;
+            // score at the window's last row B = min(m, 32 (lo + 2))
+            const uint32_t B = min(tm, 32u * (lo + 2u));
+            const bool in_b = B > 32u * (lo + 1u);
+            const uint32_t bit = (B - 1u) & 31u;
+            const uint32_t phs_ = in_b ? phb : pha, mhs_ = in_b ? mhb : mha;
+            const int32_t dsc = (int32_t)((phs_ >> bit) & 1u) - (int32_t)((mhs_ >> bit) & 1u);
+            const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
+            const uint32_t phb_s = (phb << 1) | (pha >> 31), mhb_s = (mhb << 1) | (mha >> 31);
+            if (j <= lx) {
+                score += dsc;
+                pa = mha_s | ~(xva | pha_s);
+                ma = pha_s & xva;
+                pb = mhb_s | ~(xvb | phb_s);
+                mb = phb_s & xvb;
+            }
+        }
+    }
+    const int32_t dist = tm ? score : (int32_t)lx;
+    const bool hit = act && dist <= t;
+    if (!FUSE) hits += __popcll(__ballot(hit));
+    record_hit<FUSE>(a, est, hit, x, ty, cx, tcy, core_x, tcore, lane, qacc, tacc, (uint32_t)qi, (uint32_t)tl);
+}
+
+// The batch with the banded recurrence when the pattern words and the
+// batch's largest threshold allow it, else the full one.
+template <int W, bool FUSE>
+__device__ __forceinline__ void myers_batch_any(const idb_pair_args &a, const idb_tile &tile, const uint32_t *peq,
+                                                const uint16_t *queue, uint32_t n, uint32_t lane, const idb_entry &qe,
+                                                const idb_entry &te, int32_t core_y, int32_t q_core, EdgeStage &est,
+                                                int32_t *qacc, int32_t *tacc, unsigned long long &hits,
+                                                unsigned long long &mlet)
+{
+    if (W >= 3) {
+        // the batch's largest pair threshold, uniform over the wave
+        const uint32_t e = queue[lane < n ? lane : 0u];
+        const int qi = (int)(e >> 8), tl = (int)(e & 0xFFu);
+        const uint32_t tm = (uint32_t)__shfl((int)te.len_chk, tl, 64) & 0xFFFFu;
+        const uint32_t tt2 = (uint32_t)__shfl((int)te.t2, tl, 64);
+        const uint32_t lxq = (uint32_t)__shfl((int)qe.len_chk, qi, 64) & 0xFFFFu;
+        const uint32_t qt2 = (uint32_t)__shfl((int)qe.t2, qi, 64);
+        int32_t t = lane < n ? (lxq >= tm ? t_of(qt2, FUSE) : t_of(tt2, FUSE)) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t = max(t, __shfl_xor(t, o, 64));
+        const int32_t K = __builtin_amdgcn_readfirstlane(t);
+        if (K >= 0 && K <= 15) {
+            myers_batch_band<W, FUSE>(a, tile, peq, queue, n, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
+                                      mlet, (uint32_t)K);
+            return;
+        }
+    }
+    myers_batch<W, FUSE>(a, tile, peq, queue, n, lane, qe, te, core_y, q_core, est, qacc, tacc, hits, mlet);
+}
+
 // The same for rows of at most 32 letters (one word, code rows of 16 or 32
 // bytes): each lane holds its query row's codes in registers from the start
 // of the tile (qc0, qc1), so a batch takes them from lane qi with cross-lane
@@ -680,8 +818,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         myers_batch1<FUSE>(a, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est, qacc, tacc,
                                            hits, mletters, qc0, qc1, wide);
                     else
-                        myers_batch<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est,
-                                             qacc, tacc, hits, mletters);
+                        myers_batch_any<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core,
+                                                 est, qacc, tacc, hits, mletters);
                     const uint32_t rest = qlen - IDB_TILE;
                     const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -732,8 +870,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 myers_batch1<FUSE>(a, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc, hits,
                                    mletters, qc0, qc1, wide);
             else
-                myers_batch<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc,
-                                     hits, mletters);
+                myers_batch_any<W, FUSE>(a, tile, peq, queue, qlen, lane, qe, te, core_y, q_core, est, qacc, tacc,
+                                         hits, mletters);
         }
         gated += tg;
         gletters += tgl;
@@ -761,6 +899,186 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             atomicAdd(work_copy(a) + 5, mpairs);
             atomicAdd(work_copy(a) + 6, mletters);
         }
+    }
+}
+
+// Count pass over one-word rows (at most 32 letters) of alphabets of 8-31
+// letters, outside single-bucket mode: the 200k- and 2M-row protein
+// workloads.  Lane l holds target l (its Peq column, length, threshold and
+// presence planes); the loop walks the tile's queries, two per iteration.
+// A query's gate record (length, gene check flag, count threshold, presence
+// planes, gene residues) is one 16-byte LDS word read by broadcast, so a
+// query costs its pair tests and no cross-lane traffic: the length band, the
+// triangle of a diagonal tile (a lane mask built in scalar registers), the
+// absent-letter bound (four AND / bit-count pairs) and, only for a query
+// that may share a lower gene, the canonical-bucket test.  Survivors are
+// queued in LDS and verified by Myers 64 at a time (myers_batch1).  TCONST:
+// every pair has the same count threshold (the absolute "k edits" mode), held
+// in a scalar register.
+__host__ __device__ inline size_t idb_count1_lds_per_wave(uint32_t K)
+{
+    // Peq columns, edge staging, the Myers queue, count accumulators of
+    // queries and targets, the queries' 16-byte gate records
+    return ((size_t)K * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t) + IDB_TILE * sizeof(uint4);
+}
+
+template <bool TCONST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_count1(idb_pair_args a, uint32_t wpb)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t K = a.K;
+    uint32_t *peq = lds + (size_t)wib * (idb_count1_lds_per_wave(K) / sizeof(uint32_t));
+    EdgeStage est{(uint2 *)(peq + K * 64), 0u, IDB_NONE, 0u, 0u};
+    uint16_t *queue = (uint16_t *)(peq + K * 64 + 2 * IDB_TILE);
+    int32_t *qacc = (int32_t *)(peq + K * 64 + 3 * IDB_TILE);
+    int32_t *tacc = qacc + IDB_TILE;
+    uint4 *gq = (uint4 *)(tacc + IDB_TILE);
+    const uint32_t total_waves = gridDim.x * wpb;
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + wib * gridDim.x;
+    const int32_t t_const = a.t_const;
+    unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
+
+    const uint32_t T = a.n_tiles;
+    uint32_t ti = wave0;
+    uint32_t ti1 = step_tile(ti, total_waves, T);
+    idb_tile rec1 = ti1 < T ? a.tiles[ti1] : idb_tile{};
+    TilePre<true> cur{};
+    if (ti < T) tile_load<true>(a, a.tiles[ti], lane, cur);
+    const bool wide = a.S > 16;
+    while (ti < T) {
+        TilePre<true> nxt{};
+        if (ti1 < T) tile_load<true>(a, rec1, lane, nxt);
+        const uint32_t ti2 = step_tile(ti1, total_waves, T);
+        const idb_tile rec2 = ti2 < T ? a.tiles[ti2] : idb_tile{};
+        const idb_tile tile = cur.tile;
+        const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
+        const bool diag = (tile.shape >> 16) & 1u;
+        const bool tvalid = lane < tn;
+        const idb_entry te = cur.te, qe = cur.qe;
+        const uint32_t y = te.row, m = te.len_chk & 0xFFFFu;
+        const bool chk_y = (te.len_chk >> 16) & 1u;
+        const int32_t t_m = t_of(te.t2, false);
+        // Peq of this lane's target
+        for (uint32_t c = 0; c < K; ++c) peq[c * 64 + lane] = 0u;
+        if (tvalid) {
+            const uint32_t ws[8] = {cur.tc0.x, cur.tc0.y, cur.tc0.z, cur.tc0.w,
+                                    cur.tc1.x, cur.tc1.y, cur.tc1.z, cur.tc1.w};
+#pragma unroll
+            for (uint32_t i = 0; i < 32; ++i)
+                if (i < m) atomicOr(peq + ((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) * 64 + lane, 1u << i);
+        }
+        // gate records: length | check flag << 16 | count threshold (int8) << 24,
+        // presence planes, gene residues
+        if (lane < qn) {
+            const int32_t tq = t_of(qe.t2, false);
+            const uint32_t t8 = (uint32_t)(uint8_t)(int8_t)(tq > 127 ? 127 : tq);
+            gq[lane] = make_uint4((qe.len_chk & 0x1FFFFu) | (t8 << 24), qe.pres1, qe.pres2, qe.sig);
+        }
+        qacc[lane] = 0;
+        tacc[lane] = 0;
+        const uint64_t tmask = __ballot(tvalid);
+        const uint32_t y1 = te.pres1, y2 = te.pres2, ny1 = ~te.pres1;
+        uint32_t tg = 0, tgl = 0, tmc = 0, qlen = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+
+        // One query against the tile's targets: the lanes whose pair passes
+        // the band, the triangle and the gate (vm), and of those the lanes
+        // that also pass the absent-letter bound (returned).
+        auto test = [&](uint32_t qi, const uint4 &q, uint64_t &vm) -> uint64_t {
+            const uint32_t qx = __builtin_amdgcn_readfirstlane(q.x);
+            const uint32_t lx = qx & 0xFFFFu;
+            int32_t t;
+            if (TCONST) {
+                t = t_const;
+            } else {
+                const int32_t tq = (int32_t)(int8_t)(qx >> 24);
+                t = lx >= m ? tq : t_m;
+            }
+            const int32_t dl = (int32_t)__builtin_amdgcn_sad_u16(lx, m, 0u);
+            // absent-letter bound, both directions: a letter of x whose
+            // symbol never occurs in y is substituted or deleted, one edit per
+            // position (the planes hold letters seen once / twice)
+            const uint32_t lbx = __builtin_popcount(q.y & ny1) + __builtin_popcount(q.z & ny1);
+            const uint32_t lby = __builtin_popcount(y1 & ~q.y) + __builtin_popcount(y2 & ~q.y);
+            const uint64_t lanes = diag ? (tmask & ((~0ull << qi) << 1)) : tmask;
+            vm = lanes & __ballot(dl <= t);
+            if (((qx >> 16) & 1u) && vm) {
+                // the query may share a lower gene with the target: the
+                // canonical-bucket test for targets that may too and whose
+                // residues overlap (exact test on the bitsets)
+                const bool need = __builtin_amdgcn_inverse_ballot_w64(vm) && chk_y && (q.w & te.sig) != 0u;
+                const uint64_t nm = __ballot(need);
+                if (nm) {
+                    const uint32_t x = __builtin_amdgcn_readlane(qe.row, qi);
+                    bool ok = true;
+                    if (need) ok = pair_in_bucket(a, x, y, tile.bucket);
+                    vm &= ~nm | __ballot(ok);
+                }
+            }
+            const uint32_t nv = (uint32_t)__popcll(vm);
+            tg += nv;
+            tgl += nv * lx;
+            return vm & __ballot((int32_t)(lbx > lby ? lbx : lby) <= t);
+        };
+        auto enqueue = [&](uint32_t qi, uint64_t pm) {
+            if (!pm) return;
+            if (__builtin_amdgcn_inverse_ballot_w64(pm)) {
+                const uint32_t below =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pm, 0u));
+                queue[qlen + below] = (uint16_t)((qi << 8) | lane);
+            }
+            const uint32_t np = (uint32_t)__popcll(pm);
+            qlen += np;
+            tmc += np;
+            if (qlen >= IDB_TILE) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                myers_batch1<false>(a, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters,
+                                    cur.qc0, cur.qc1, wide);
+                const uint32_t rest = qlen - IDB_TILE;
+                const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                if (lane < rest) queue[lane] = moved;
+                qlen = rest;
+            }
+        };
+        // two queries per iteration, both records read before either is
+        // used (gq[qi + 1] is inside the 64 records even past qn)
+        for (uint32_t qi = 0; qi < qn; qi += 2) {
+            const uint4 q0 = gq[qi], q1 = gq[qi + 1];
+            uint64_t v0 = 0, v1 = 0;
+            const uint64_t p0 = test(qi, q0, v0);
+            const uint64_t p1 = qi + 1 < qn ? test(qi + 1, q1, v1) : 0ull;
+            enqueue(qi, p0);
+            enqueue(qi + 1, p1);
+        }
+        if (qlen) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            myers_batch1<false>(a, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters, cur.qc0,
+                                cur.qc1, wide);
+        }
+        gated += tg;
+        gletters += tgl;
+        mpairs += tmc;
+        // one atomic per row and tile
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const int32_t qa = qacc[lane], ta = tacc[lane];
+        if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
+        if (tvalid && ta) atomicAdd(a.cnt + y, ta);
+        cur = nxt;
+        ti = ti1;
+        ti1 = ti2;
+        rec1 = rec2;
+    }
+    finish_edges(a, est, lane);
+    mletters = wave_sum_u64(mletters);
+    if (lane == 0 && a.work) {
+        atomicAdd(work_copy(a) + 0, gated);
+        atomicAdd(work_copy(a) + 1, hits);
+        atomicAdd(work_copy(a) + 3, gletters);
+        atomicAdd(work_copy(a) + 5, mpairs);
+        atomicAdd(work_copy(a) + 6, mletters);
     }
 }
 
@@ -1257,6 +1575,46 @@ __global__ void k_union_compress(const uint32_t *n_chunks, uint32_t max_chunks, 
     }
 }
 
+// Depth histogram of the union-find forest as cluster growth left it, before
+// the roots are collected (the reference's tree_depth_evaluation_kernel,
+// cuda/PSDBSCAN.hcu:154-177, run after grow_clusters at :1322-1325): a
+// duplicate row is one step from its representative, a border row one step
+// from the core row that claimed it, and a core row as many steps as its
+// entry is below the root of the entry forest.  The depths depend on the
+// linking order, as the reference's do.
+__global__ void k_union_depth(const uint32_t *n_chunks, uint32_t max_chunks, const uint32_t *rep, const uint32_t *pe,
+                              const uint32_t *bmin, const uint32_t *par_e, uint32_t n_rows, uint32_t *hist)
+{
+    if (pairs_overflowed(n_chunks, max_chunks)) return;
+    __shared__ uint32_t h[51];
+    for (uint32_t k = threadIdx.x; k < 51; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_rows; i += gridDim.x * blockDim.x) {
+        uint32_t d = 0, x = i;
+        if (rep[x] != x) {
+            x = rep[x];
+            ++d;
+        }
+        uint32_t f = pe[x];
+        if ((f == IDB_NONE || !(f & PE_CORE)) && bmin[x] != IDB_NONE) {
+            x = bmin[x];
+            ++d;
+            f = pe[x];
+        }
+        if (f != IDB_NONE && (f & PE_CORE)) {
+            uint32_t e = f & ~PE_CORE;
+            for (uint32_t p = ld_cached(par_e + e); p != e && d < 50; p = ld_cached(par_e + e)) {
+                e = p;
+                ++d;
+            }
+        }
+        atomicAdd(&h[d > 50 ? 50 : d], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 51; k += blockDim.x)
+        if (h[k]) atomicAdd(hist + k, h[k]);
+}
+
 __global__ void k_union_comp_min(const uint32_t *n_chunks, uint32_t max_chunks, const uint32_t *pe,
                                  const uint32_t *par_e, uint32_t *cmin_e, uint32_t n_rows)
 {
@@ -1334,6 +1692,31 @@ hipError_t launch_myers_f(const idb_pair_args &a, uint32_t W, dim3 block, size_t
     }
 }
 
+template <bool TCONST>
+hipError_t launch_count1_t(const idb_pair_args &a, uint32_t wpb, hipStream_t s)
+{
+    const size_t shm = idb_count1_lds_per_wave(a.K) * wpb;
+    const void *k = (const void *)k_count1<TCONST>;
+    if (shm > 65536) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm), a.tune_grid_mul));
+    hipLaunchKernelGGL(k_count1<TCONST>, grid, block, shm, s, a, wpb);
+    return hipGetLastError();
+}
+
+// The one-word count pass (k_count1); the caller has filled the launch shape.
+hipError_t launch_count1(const idb_pair_args &a, hipStream_t s)
+{
+    const size_t per_wave = idb_count1_lds_per_wave(a.K);
+    uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
+    if (wpb > 4) wpb = 4;
+    if (wpb == 0) return hipErrorInvalidValue;
+    wpb = pick_wpb(a, wpb);
+    return a.t_const >= 0 ? launch_count1_t<true>(a, wpb, s) : launch_count1_t<false>(a, wpb, s);
+}
+
 template <bool FUSE>
 hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
 {
@@ -1361,6 +1744,7 @@ hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
     const size_t shm = per_wave * wpb;
     if (W == 1 && a.band_filter) {
         if (a.S > 32 || a.K > 32) return hipErrorInvalidValue;  // one-word rows, alphabets of at most 31 letters
+        if (!FUSE && !a.single_bucket) return launch_count1(a, s);
         return launch_myers_w<1, FUSE, 1>(a, block, shm, wpb, s);
     }
     if (a.qgram_filter) return launch_myers_f<FUSE, 2>(a, W, block, shm, wpb, s);
@@ -1494,6 +1878,9 @@ hipError_t idb_launch_union_edges(const idb_pair_args &a, const idb_union_args &
     hipLaunchKernelGGL(k_union_entries, dim3(b2), dim3(256), 0, s, a.edges, a.chunk_fill, a.n_chunks, a.max_chunks,
                        a.chunk_log2, u.par_e);
     const uint32_t rb = (u.n_rows + 255) / 256 < 2048u ? (u.n_rows + 255) / 256 : 2048u;
+    if (u.depth_hist)
+        hipLaunchKernelGGL(k_union_depth, dim3(rb ? rb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.rep, u.pe,
+                           u.bmin, u.par_e, u.n_rows, u.depth_hist);
     hipLaunchKernelGGL(k_union_comp_min, dim3(rb ? rb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.pe, u.par_e,
                        u.cmin_e, u.n_rows);
     hipLaunchKernelGGL(k_union_to_rows, dim3(rb ? rb : 1u), dim3(256), 0, s, a.n_chunks, a.max_chunks, u.pe, u.par_e,

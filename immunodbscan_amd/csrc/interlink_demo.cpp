@@ -3,10 +3,12 @@
 // three string columns and two gene lists in, clone_id / copies / cores out.
 //
 // usage: interlink_demo <rows.tsv> <V genes file> <J genes file> <eps> <minPts> <hamming 0|1>
+//                       [<run log .tsv> [<tree stats 0|1>]]
 //   rows.tsv   one row per line: V call <TAB> J call <TAB> sequence
 //   gene files one gene name per line, empty lines skipped (cpp/GeneReader.hpp:65-76)
 // Output: one line per row, "clone_id<TAB>copies<TAB>cores", then
-// "# n_uniques <k>" and "# v_bitset_width <w>".
+// "# n_uniques <k>" and "# v_bitset_width <w>"; with a run-log path, the
+// run's messages and stage timers in the reference's TSV format.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -16,6 +18,7 @@
 #include <string>
 
 #include "Interlink.hpp"
+#include "RunLog.hpp"
 
 static thrust::host_vector<std::string> read_genes(const std::string &path)
 {
@@ -30,8 +33,8 @@ static thrust::host_vector<std::string> read_genes(const std::string &path)
 
 int main(int argc, char **argv)
 {
-    if (argc != 7) {
-        std::fprintf(stderr, "usage: %s rows.tsv Vgenes Jgenes eps minPts hamming\n", argv[0]);
+    if (argc < 7 || argc > 9) {
+        std::fprintf(stderr, "usage: %s rows.tsv Vgenes Jgenes eps minPts hamming [runlog.tsv [tree_stats]]\n", argv[0]);
         return 2;
     }
     try {
@@ -55,13 +58,16 @@ int main(int argc, char **argv)
         const bool ham = std::atoi(argv[6]) != 0;
         NormalStringVectorZipIteratorType zip =
             thrust::make_zip_iterator(thrust::make_tuple(v.begin(), j.begin(), s.begin()));
+        const bool trees = argc > 8 && std::atoi(argv[8]) != 0;
         InterlinkClass il(zip, (unsigned)s.size(), vg.begin(), (unsigned)vg.size(), jg.begin(), (unsigned)jg.size(), eps,
-                          minpts, ham, false);
+                          minpts, ham, trees);
         auto cl = il.get_clusters_iterator();
         auto cp = il.get_copies_iterator();
         auto co = il.get_cores_iterator();
         for (size_t i = 0; i < s.size(); ++i) std::printf("%u\t%u\t%d\n", cl[i], cp[i], co[i]);
         std::printf("# n_uniques %u\n# v_bitset_width %u\n", il.get_n_uniques(), il.get_v_bitset_width());
+        if (argc > 7 && !idb::RunLog::instance().write(argv[7]))
+            throw std::runtime_error(std::string("Cannot write to file ") + argv[7]);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -279,8 +279,9 @@ def test_interlink_cpp_drop_in(gpu_ctx, tmp_path):
             f.write(", or ".join(f"Homsap {vh[g][:-1]}*01 F" for g in v) + "\t"
                     + ", or ".join(f"Homsap {jh[g][:-1]}*01 F" for g in j) + "\t" + s.decode() + "\n")
     for eps, minpts, ham in ((0.15, 2, 1), (0.1, 3, 0)):
+        runlog = tmp_path / f"runlog_{ham}.tsv"
         out = subprocess.run([exe, str(tsv), "tests/golden/VHgenes.txt", "tests/golden/JHgenes.txt", str(eps),
-                              str(minpts), str(ham)], capture_output=True, text=True, timeout=120)
+                              str(minpts), str(ham), str(runlog), "1"], capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, out.stderr
         lines = [ln for ln in out.stdout.split("\n") if ln and not ln.startswith("#")]
         got = np.array([[int(x) for x in ln.split("\t")] for ln in lines])
@@ -297,6 +298,26 @@ def test_interlink_cpp_drop_in(gpu_ctx, tmp_path):
         assert np.array_equal(got[:, 1], exp["copies"])
         assert np.array_equal(got[:, 2], exp["core"])
         assert f"# n_uniques {exp['n_uniques']}" in out.stdout
+        # the run log in the reference's BenchmarkSingleton format
+        # (cpp/BenchmarkSingleton.hpp:104-115, messages of PSDBSCAN.hcu:986-997,1113,1198,1279-1292)
+        lines = runlog.read_text().split("\n")
+        assert lines[0] == "Timer/Message\tValue"
+        log_rows = [ln.split("\t") for ln in lines[1:] if ln]
+        msgs = [b for a, b in log_rows if a.startswith("At ") and a.endswith(" sec")]
+        timers = {a: b for a, b in log_rows if not a.startswith("At ")}
+        assert msgs[0] == f"DBSCAN settings: epsilon = {eps:.6f}" and msgs[1] == f"DBSCAN settings: minPts = {minpts}"
+        assert ("DBSCAN settings: using Hamming distance measure instead of Levenshtein distance" in msgs) == bool(ham)
+        assert f"Total number of sequences = {len(got)}" in msgs
+        assert f"Number of uniques = {exp['n_uniques']}" in msgs
+        assert f"Number of core points = {exp['n_cores']}" in msgs
+        assert f"Total number of clusters = {int(exp['clone_id'].max())}" in msgs
+        n_noise = int((exp["clone_id"] == 0).sum())
+        assert f"Number of noise points = {n_noise}" in msgs
+        assert f"Percentage of noise points = {100 * n_noise // len(got)}%" in msgs
+        depths = [int(m.split(": ")[1]) for m in msgs if m.startswith("Number of trees with depth ")]
+        assert len(depths) == 51 and sum(depths) == len(got)
+        for name in _lib.STAGE_NAMES:
+            assert timers[name].endswith(" sec") and float(timers[name][:-4]) >= 0.0
 
 
 def test_large_input_paths_forced_on_small_inputs(gpu_ctx):
@@ -321,3 +342,23 @@ def test_large_input_paths_forced_on_small_inputs(gpu_ctx):
         exp = oracle_of(rp, configs.C3)
         for k in ("clone_id", "core", "copies", "uniques", "counts"):
             assert np.array_equal(getattr(got, k)[s:e], exp[k]), (r, k)
+
+
+def test_tree_depth_histogram_measures_the_forest_before_compression(gpu_ctx):
+    """The depth histogram (the reference's tree_depth_evaluation_kernel,
+    PSDBSCAN.hcu:154-177, run right after cluster growth) on a forest whose
+    shape is fixed: row 0 (100 letters) is a star centre one substitution from
+    100 distinct variants (two apart from each other), so every variant is a core row
+    linked straight under row 0, and 5 duplicates of the first variant are
+    two steps down; from the pair list and from the fuse pass after a
+    pair-list overflow (100 pairs, capacity 64) alike."""
+    variants = ["A" * p + "C" + "A" * (99 - p) for p in range(100)]
+    rows = [("A" * 100, [3], [1])] + [(v, [3], [1]) for v in variants] + [(variants[0], [3], [1])] * 5
+    rep = rep_from_rows(rows)
+    cl = configs.Clustering(eps=0.0, abs_k=1, min_pts=2)
+    for cap in (0, 1):
+        got = engine.dbscan(rep, cl, tree_stats=True, edge_capacity=cap)
+        assert got.stats["fuse_from_edges"] == (1 if cap == 0 else 0)
+        hist = got.stats["tree_depth"]
+        assert list(hist[:4]) == [1, 100, 5, 0], (cap, list(hist[:6]))
+        assert sum(hist) == rep.n
