@@ -401,58 +401,83 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
     uint32_t pa = 0xFFFFFFFFu, ma = 0u, pb = 0xFFFFFFFFu, mb = 0u;
     int32_t score = (int32_t)tm;
     uint32_t lo = 0;
-    const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
-    const uint32_t *pcol = peq + tl;
     uint32_t lmax = lx;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, (uint32_t)__shfl_xor((int)lmax, o, 64));
     lmax = __builtin_amdgcn_readfirstlane(lmax);
-    for (uint32_t j0 = 0; j0 < lmax; j0 += 4) {
-        const uint32_t word = j0 < lx ? xc[j0 >> 2] : 0u;
+    // Peq address of letter c in the window: c * (W * 64) + lo * 64 + tl words
+    uint32_t pbase = (uint32_t)tl;
+    // the tracked row B = min(m, 32 (lo + 2)): in word b or word a, at bit
+    // (B - 1) & 31; fixed while the window stays
+    bool in_b = false;
+    uint32_t bit = 0;
+    auto place = [&]() {
+        const uint32_t B = min(tm, 32u * (lo + 2u));
+        in_b = B > 32u * (lo + 1u);
+        bit = (B - 1u) & 31u;
+    };
+    place();
+    // Sixteen letters of the query per step group, from two register
+    // buffers: while one group runs, the next-but-one is loaded into the
+    // buffer just used (S is a multiple of 16, so every load is in the row).
+    auto group = [&](const uint4 &cur, uint32_t j0) {
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
-            const uint32_t j = j0 + k + 1;  // 1-based text column
-            if (j > lmax) break;
-            // window of column j: the word of row j - K, kept within [0, W - 2]
-            const uint32_t want = j > K + 1 ? min((j - K - 1) >> 5, (uint32_t)W - 2u) : 0u;
-            if (want != lo) {  // uniform: one word further down
-                pa = pb;
-                ma = mb;
-                pb = 0xFFFFFFFFu;
-                mb = 0u;
-                lo = want;
-            }
-            const uint32_t c = (word >> (8 * k)) & 0xFFu;
-            const uint32_t ea = pcol[(c * W + lo) * 64], eb = pcol[(c * W + lo + 1) * 64];
-            // word a: boundary row above it (shift-in +1, no add carry)
-            const uint32_t xva = ea | ma;
-            const uint64_t sa64 = (uint64_t)(ea & pa) + pa;
-            const uint32_t xha = ((uint32_t)sa64 ^ pa) | ea;
-            const uint32_t pha = ma | ~(xha | pa);
-            const uint32_t mha = pa & xha;
-            // word b: carries from word a
-            const uint32_t cin = (uint32_t)(sa64 >> 32);
-            const uint32_t xvb = eb | mb;
-            const uint32_t sb = (eb & pb) + pb + cin;
-            const uint32_t xhb = (sb ^ pb) | eb;
-            const uint32_t phb = mb | ~(xhb | pb);
-            const uint32_t mhb = pb & xhb;
-            // score at the window's last row B = min(m, 32 (lo + 2))
-            const uint32_t B = min(tm, 32u * (lo + 2u));
-            const bool in_b = B > 32u * (lo + 1u);
-            const uint32_t bit = (B - 1u) & 31u;
-            const uint32_t phs_ = in_b ? phb : pha, mhs_ = in_b ? mhb : mha;
-            const int32_t dsc = (int32_t)((phs_ >> bit) & 1u) - (int32_t)((mhs_ >> bit) & 1u);
-            const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
-            const uint32_t phb_s = (phb << 1) | (pha >> 31), mhb_s = (mhb << 1) | (mha >> 31);
-            if (j <= lx) {
-                score += dsc;
-                pa = mha_s | ~(xva | pha_s);
-                ma = pha_s & xva;
-                pb = mhb_s | ~(xvb | phb_s);
-                mb = phb_s & xvb;
+        for (uint32_t q = 0; q < 4; ++q) {
+            if (j0 + 4 * q >= lmax) break;  // uniform
+            const uint32_t word = q == 0 ? cur.x : (q == 1 ? cur.y : (q == 2 ? cur.z : cur.w));
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t j = j0 + 4 * q + k + 1;  // 1-based text column (past lmax: no lane updates)
+                // window of column j: the word of row j - K, kept within [0, W - 2]
+                const uint32_t want = j > K + 1 ? min((j - K - 1) >> 5, (uint32_t)W - 2u) : 0u;
+                if (want != lo) {  // uniform: one word further down
+                    pa = pb;
+                    ma = mb;
+                    pb = 0xFFFFFFFFu;
+                    mb = 0u;
+                    lo = want;
+                    pbase = (uint32_t)tl + lo * 64u;
+                    place();
+                }
+                const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                const uint32_t *pe = peq + c * (W * 64u) + pbase;
+                const uint32_t ea = pe[0], eb = pe[64];
+                // word a: boundary row above it (shift-in +1, no add carry)
+                const uint32_t xva = ea | ma;
+                const uint64_t sa64 = (uint64_t)(ea & pa) + pa;
+                const uint32_t xha = ((uint32_t)sa64 ^ pa) | ea;
+                const uint32_t pha = ma | ~(xha | pa);
+                const uint32_t mha = pa & xha;
+                // word b: carries from word a
+                const uint32_t cin = (uint32_t)(sa64 >> 32);
+                const uint32_t xvb = eb | mb;
+                const uint32_t sb = (eb & pb) + pb + cin;
+                const uint32_t xhb = (sb ^ pb) | eb;
+                const uint32_t phb = mb | ~(xhb | pb);
+                const uint32_t mhb = pb & xhb;
+                const uint32_t phs_ = in_b ? phb : pha, mhs_ = in_b ? mhb : mha;
+                const int32_t dsc = (int32_t)((phs_ >> bit) & 1u) - (int32_t)((mhs_ >> bit) & 1u);
+                const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
+                const uint32_t phb_s = (phb << 1) | (pha >> 31), mhb_s = (mhb << 1) | (mha >> 31);
+                if (j <= lx) {
+                    score += dsc;
+                    pa = mha_s | ~(xva | pha_s);
+                    ma = pha_s & xva;
+                    pb = mhb_s | ~(xvb | phb_s);
+                    mb = phb_s & xvb;
+                }
             }
         }
+    };
+    const uint4 *xq = (const uint4 *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    uint4 bufa = lx ? xq[0] : zero4, bufb = lx > 16 ? xq[1] : zero4;
+    for (uint32_t j0 = 0; j0 < lmax; j0 += 32) {
+        group(bufa, j0);
+        bufa = j0 + 32 < lx ? xq[(j0 >> 4) + 2] : zero4;
+        if (j0 + 16 >= lmax) break;
+        group(bufb, j0 + 16);
+        bufb = j0 + 48 < lx ? xq[(j0 >> 4) + 3] : zero4;
     }
     const int32_t dist = tm ? score : (int32_t)lx;
     const bool hit = act && dist <= t;
