@@ -7,8 +7,9 @@ c2  200,000 aa CDR3s, 20,000 seeds (lognormal mu=1 sigma=1.2, scaled to the
     row count), lengths round(N(15, 3)) clipped to 8-25, 1-3 edits per member,
     64 V genes Zipf(1.1) (second call p=0.2), 8 J genes; Levenshtein, t = 2
     edits, minPts = 3.  Seed 2.
-c3  512 repertoires, sizes lognormal (median 5,000, sigma 1.65, clipped to
-    200-200,000), each generated as c1 with size/10 seeds and Zipf(1.1) V
+c3  512 repertoires, sizes lognormal (median 5,000, sigma 1.82, clipped to
+    200-200,000: sigma chosen so that the seed-3 draw totals about 10M rows,
+    10,018,569), each generated as c1 with size/10 seeds and Zipf(1.1) V
     usage.  Seed 3 (repertoire r uses seed 3 * 1000003 + r).
 c4  2,000,000 aa CDR3s generated as c2 with 200,000 seeds (the encoder of the
     learned filter works on amino acids).  Seed 4.
@@ -79,7 +80,7 @@ C2 = Clustering(eps=0.0, abs_k=2, min_pts=3, use_hamming=False)
 
 
 def c3_sizes(n_reps: int = 512, seed: int = 3) -> np.ndarray:
-    return synth.lognormal_sizes(seed, n_reps, 5000.0, 1.65, 200, 200_000)
+    return synth.lognormal_sizes(seed, n_reps, 5000.0, 1.82, 200, 200_000)
 
 
 def c3_repertoires(n_reps: int = 512, seed: int = 3, scale: float = 1.0) -> list[synth.Repertoire]:

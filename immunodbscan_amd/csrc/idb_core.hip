@@ -1400,7 +1400,9 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     uint32_t *ustart = n_reps > 0 ? buf<uint32_t>(c, S_USTART, n_reps + 1) : nullptr;
     reserve_scan_tmp(c, (size_t)N + 1);
     if (E > 0 && T > 0 && (p->flags & IDB_FLAG_AUTOTUNE)) autotune(c, a, N, W, ham, learned, small, work);
-    const bool graph = (p->flags & IDB_FLAG_GRAPH) && !(x && x->pairs_out);
+    // Learned runs stay uncaptured: when their pair list overflows, the host
+    // grows it and runs the learned pass again (below).
+    const bool graph = (p->flags & IDB_FLAG_GRAPH) && !(x && x->pairs_out) && !learned;
     bool from_edges = false;
 
     // Everything after the sizing steps.  As a graph (g) the union / fuse
@@ -1443,9 +1445,31 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             } else {
                 uint32_t n_chunks = 0;
                 readback(c, &n_chunks, small + 12, sizeof(n_chunks));
+                for (int tries = 0; learned && n_chunks > a.max_chunks && tries < 3; ++tries) {
+                    // The learned pass found more neighbour pairs than the
+                    // list holds.  A second pass over the tiles would have
+                    // to repeat its proposals, so instead the list grows to
+                    // what the pass claimed (plus a margin: the claim order
+                    // of a new run differs) and the pass runs again from its
+                    // starting state: counts back to each row's own copies,
+                    // chunk and statistics counters cleared.  The counts of
+                    // the first pass were complete, so the core flags stand.
+                    const uint64_t want = (uint64_t)n_chunks + n_chunks / 4 + 64;
+                    a.max_chunks = (uint32_t)std::min<uint64_t>(want, (1ull << 31) >> a.chunk_log2);
+                    a.edges = buf<uint2>(c, S_EDGES, (size_t)a.max_chunks << a.chunk_log2);
+                    a.chunk_fill = buf<uint32_t>(c, S_FILL, a.max_chunks);
+                    check(c, hipMemcpyAsync(cnt, copies, sizeof(int32_t) * N, hipMemcpyDeviceToDevice, s), "count reset");
+                    check(c, hipMemsetAsync(small + 12, 0, 4 * sizeof(uint32_t), s), "chunk reset");
+                    check(c, hipMemsetAsync(work, 0, IDB_WORK_U64 * IDB_WORK_COPIES * sizeof(unsigned long long), s),
+                          "statistics reset");
+                    check(c, hipEventRecord(c->kev[0], s), "event record");
+                    check(c, idb_launch_count_learned(a, s), "learned count kernel launch");
+                    check(c, hipEventRecord(c->kev[1], s), "event record");
+                    readback(c, &n_chunks, small + 12, sizeof(n_chunks));
+                }
                 from_edges = n_chunks <= a.max_chunks;
                 if (learned && !from_edges)
-                    fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode: raise edge_capacity");
+                    fail(c, IDB_ERR_OOM, "neighbour pair list overflow in learned mode after growing it");
                 if (x && x->pairs_out) {
                     if (!from_edges) fail(c, IDB_ERR_OOM, "neighbour pair list overflow: raise edge_capacity");
                     std::vector<uint32_t> fill(n_chunks);

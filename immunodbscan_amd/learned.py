@@ -127,7 +127,7 @@ def neighbour_pairs(rep: Repertoire, cl: Clustering, ctx=None, cap: int | None =
 
 
 def dbscan_learned(rep: Repertoire, cl: Clustering, enc: Encoder, min_sim: float, ctx=None,
-                   counts: bool = True) -> Result:
+                   counts: bool = True, edge_capacity: int = 0) -> Result:
     ctx = ctx or default_context()
     n = rep.n
     cid = np.zeros(n, np.uint32)
@@ -138,7 +138,7 @@ def dbscan_learned(rep: Repertoire, cl: Clustering, enc: Encoder, min_sim: float
     outs = _lib.make_outputs(cid, core, cop, unq, cnt)
     st = _lib.IdbStats()
     ctx.check(_lib.lib.idb_dbscan_learned(ctx.handle, enc.handle, float(min_sim), _dataset(rep),
-                                          _params(cl, False, 0), outs, st))
+                                          _params(cl, False, 0, edge_capacity), outs, st))
     return Result(cid, core, cop, unq, int(st.n_uniques), cnt, st.as_dict())
 
 

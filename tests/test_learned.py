@@ -163,6 +163,14 @@ def test_learned_run_proposing_everything_equals_exact(gpu_ctx):
     # every gated pair proposed; the same exact letter bound before Myers
     assert le.stats["gated_pairs"] == ex.stats["gated_pairs"]
     assert le.stats["myers_pairs"] == ex.stats["myers_pairs"]
+    # a pair list far too short: the learned pass runs again with a grown
+    # list, with the same results and statistics
+    sm = learned.dbscan_learned(rep, configs.C2, enc, -2.0, edge_capacity=64)
+    assert sm.stats["fuse_from_edges"] == 1
+    for f in ("clone_id", "core", "copies", "counts"):
+        assert np.array_equal(getattr(sm, f), getattr(le, f)), f
+    for k in ("gated_pairs", "neighbour_pairs", "myers_pairs"):
+        assert sm.stats[k] == le.stats[k], k
 
 
 @pytest.mark.gpu
