@@ -3,6 +3,7 @@
     libidb_hip.so    HIP kernels + C ABI (hipcc, gfx950)
     libidb_synth.so  synthetic repertoire generator (g++)
     interlink_demo   C++ drop-in InterlinkClass driver (hipcc host code)
+    psdbscan         the host program: CSV + INFO config in, CSV + run log out
 
 Run ``python -m immunodbscan_amd.build`` or call :func:`build_all`.
 """
@@ -75,13 +76,25 @@ def build_interlink() -> str:
     return out
 
 
+def build_psdbscan() -> str:
+    """The host program over the drop-in InterlinkClass: reads CSV files and
+    an INFO configuration, writes the clustered CSV and the run log."""
+    out = os.path.join(PKG, "psdbscan")
+    _run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+          os.path.join(CSRC, "Interlink.cpp"), os.path.join(CSRC, "psdbscan.cpp"),
+          "-L" + PKG, "-lidb_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(interlink: bool = True) -> None:
     build_synth()
     build_hip()
     if interlink and os.path.exists(os.path.join(CSRC, "Interlink.cpp")):
         build_interlink()
+        build_psdbscan()
 
 
 if __name__ == "__main__":
     build_all()
-    print("built:", [f for f in os.listdir(PKG) if f.endswith(".so") or f == "interlink_demo"])
+    print("built:", [f for f in os.listdir(PKG) if f.endswith(".so") or f in ("interlink_demo", "psdbscan")])

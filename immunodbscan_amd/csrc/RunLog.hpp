@@ -11,7 +11,9 @@
 // 1279-1292, 1300-1354), with the library's stage times.
 #pragma once
 #include <chrono>
+#include <cstdio>
 #include <fstream>
+#include <map>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -32,13 +34,38 @@ class RunLog {
         std::lock_guard<std::mutex> lock(mu_);
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
         rows_.emplace_back("At " + std::to_string(s) + " sec", text);
+        if (verbose_) std::printf("\n%s", text.c_str());
     }
 
     void timer(const std::string &name, double seconds)
     {
         std::lock_guard<std::mutex> lock(mu_);
         rows_.emplace_back(name, std::to_string(seconds) + " sec");
+        if (verbose_) std::printf("\nTimer %s: %f sec", name.c_str(), seconds);
     }
+
+    // Named timers: start (or restart) one, stop it to add its row; a timer
+    // never started adds nothing.
+    void start(const std::string &name)
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        starts_[name] = std::chrono::steady_clock::now();
+    }
+
+    void stop(const std::string &name)
+    {
+        std::chrono::steady_clock::time_point t0;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            const auto it = starts_.find(name);
+            if (it == starts_.end()) return;
+            t0 = it->second;
+        }
+        timer(name, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+
+    // Messages and timers are also printed as they arrive.
+    void be_verbose() { verbose_ = true; }
 
     std::vector<std::pair<std::string, std::string>> rows() const
     {
@@ -61,6 +88,8 @@ class RunLog {
     mutable std::mutex mu_;
     std::chrono::steady_clock::time_point t0_;
     std::vector<std::pair<std::string, std::string>> rows_;
+    std::map<std::string, std::chrono::steady_clock::time_point> starts_;
+    bool verbose_ = false;
 };
 
 }  // namespace idb
