@@ -573,7 +573,24 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
             for (uint32_t w = 0; w < IDB_QGRAM_WORDS / 4; ++w)
                 dst[w] = make_uint4(row[4 * w], row[4 * w + 1], row[4 * w + 2], row[4 * w + 3]);
         }
-        if (epk) {
+        if (epk && bits == 5) {
+            // six 5-bit letters per word (bits 0..29), trailing fields zero
+            uint32_t *dst = epk + (size_t)e * HW;
+            uint32_t cur = 0, fld = 0, wi = 0;
+            for (uint32_t k0 = 0; k0 < L; k0 += 16) {
+                const uint4 v = row16(s, L, k0);
+                for (uint32_t b = 0; b < 16 && k0 + b < L; ++b) {
+                    cur |= (uint32_t)cm[byte_of(v, b)] << (5 * fld);
+                    if (++fld == 6) {
+                        dst[wi++] = cur;
+                        cur = 0;
+                        fld = 0;
+                    }
+                }
+            }
+            if (fld) dst[wi++] = cur;
+            for (; wi < HW; ++wi) dst[wi] = 0u;
+        } else if (epk) {
             // 16 letters per step: one 2-bit word or four 8-bit words
             uint32_t *dst = epk + (size_t)e * HW;
             const uint32_t per = 32 / bits;
@@ -1169,10 +1186,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     const bool ham = p->use_hamming != 0;
     uint32_t W = 0, S = 0, HW = 0, bits = 0;
     if (ham) {
-        bits = K <= 4 ? 2u : 8u;
-        HW = std::max(1u, (maxlen * bits + 31) / 32);
+        // letter codes 0..K-1 in 2-, 5- or 8-bit fields: 16, 6 or 4 per word
+        bits = K <= 4 ? 2u : (K <= 32 ? 5u : 8u);
+        const uint32_t per = 32 / bits;
+        HW = std::max(1u, (maxlen + per - 1) / per);
         if (HW > IDB_MAX_HW)
-            fail(c, IDB_ERR_UNSUPPORTED, "Hamming rows longer than " + std::to_string(IDB_MAX_HW * 32 / bits) +
+            fail(c, IDB_ERR_UNSUPPORTED, "Hamming rows longer than " + std::to_string(IDB_MAX_HW * per) +
                                              " letters are not supported for this alphabet");
     } else {
         W = std::max(1u, (maxlen + 31) / 32);

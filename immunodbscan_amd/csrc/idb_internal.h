@@ -103,7 +103,7 @@ struct idb_pair_args {
     const uint8_t *e_codes;       // Myers: stride S bytes of codes
     const uint32_t *e_pk;         // Hamming: stride HW packed words
     uint32_t S, HW, K;
-    uint32_t bits;                // Hamming code width: 2 or 8
+    uint32_t bits;                // Hamming code width: 2, 5 or 8
     const int32_t *vbits, *jbits;
     uint32_t vw, jw;
     uint32_t gj;                  // J genes per bucket row (32*jw)

@@ -1408,14 +1408,16 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
             if (!vm) continue;
             gated += __popcll(vm);
             gletters += (unsigned long long)__popcll(vm) * lx;
-            const uint32_t nw = (lx * BITS + 31) / 32;
+            constexpr uint32_t PER = 32 / BITS;  // letters per word
+            const uint32_t nw = (lx + PER - 1) / PER;
             const uint32_t *xp = a.e_pk + (size_t)(tile.q0 + qi) * HW;
             uint32_t ham = 0;
 #pragma unroll
             for (int w = 0; w < HWMAX; ++w) {
                 if ((uint32_t)w < nw) {
                     const uint32_t xw = QREG ? rl(qw[QREG ? w : 0], qi) : xp[w];
-                    ham += BITS == 2 ? idb_mismatch2(xw, tw[w]) : idb_mismatch8(xw, tw[w]);
+                    ham += BITS == 2 ? idb_mismatch2(xw, tw[w])
+                                     : (BITS == 5 ? idb_mismatch5(xw, tw[w]) : idb_mismatch8(xw, tw[w]));
                 }
             }
             // count pass: the threshold is compared only after a mismatch
@@ -1810,7 +1812,9 @@ template <bool FUSE>
 hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
 {
     if (a.n_tiles == 0) return hipSuccess;
-    return a.bits == 2 ? launch_hamming_bits<2, FUSE>(a, s) : launch_hamming_bits<8, FUSE>(a, s);
+    if (a.bits == 2) return launch_hamming_bits<2, FUSE>(a, s);
+    if (a.bits == 5) return launch_hamming_bits<5, FUSE>(a, s);
+    return launch_hamming_bits<8, FUSE>(a, s);
 }
 
 hipError_t launch_learned(const idb_pair_args &a_in, hipStream_t s)

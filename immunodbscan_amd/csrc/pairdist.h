@@ -76,6 +76,17 @@ IDB_HD uint32_t idb_mismatch2(uint32_t a, uint32_t b)
     return (uint32_t)__builtin_popcount((x | (x >> 1)) & 0x55555555u);
 }
 
+// Six 5-bit fields (bits 0..29): adding 0b01111 to a field's low four bits
+// carries into its top bit exactly when they are not all zero, and never
+// into the next field (15 + 15 < 32); or-ing the field's own top bit makes
+// the top bit "the field differs".
+IDB_HD uint32_t idb_mismatch5(uint32_t a, uint32_t b)
+{
+    const uint32_t x = a ^ b;
+    const uint32_t y = ((x & 0x1EF7BDEFu) + 0x1EF7BDEFu) | x;
+    return (uint32_t)__builtin_popcount(y & 0x21084210u);
+}
+
 IDB_HD uint32_t idb_mismatch8(uint32_t a, uint32_t b)
 {
     const uint32_t x = a ^ b;

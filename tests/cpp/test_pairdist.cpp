@@ -89,6 +89,12 @@ int main()
             a2 |= c << (2 * i); b2 |= d << (2 * i); e2 += c != d;
         }
         if (idb_mismatch2(a2, b2) != e2) ++bad;
+        uint32_t a5 = 0, b5 = 0, e5 = 0;
+        for (int i = 0; i < 6; ++i) {
+            uint32_t c = rnd(32), d = rnd(2) ? c : rnd(32);
+            a5 |= c << (5 * i); b5 |= d << (5 * i); e5 += c != d;
+        }
+        if (idb_mismatch5(a5, b5) != e5) ++bad;
     }
     std::printf("%s mismatches=%d\n", bad ? "FAIL" : "OK", bad);
     return bad ? 1 : 0;

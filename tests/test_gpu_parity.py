@@ -104,6 +104,18 @@ def test_lengths(gpu_ctx, lo, hi, hamming):
     check(rep_from_rows(rows), cl, tag=f"len {lo}-{hi}")
 
 
+def test_amino_acid_hamming_long_rows(gpu_ctx):
+    """Alphabets of 5..32 letters pack six 5-bit letters per word; rows of up
+    to 384 letters fit the 64 words a row may have."""
+    rng = random.Random(77)
+    rows = random_rows(rng, 1200, b"ACDEFGHIKLMNPQRSTVWY", 250, 380, 3, 2, 0.2, families=60, edits=3)
+    rows += [(b"W" * 384, [0], [0]), (b"W" * 383 + b"Y", [0], [0])]
+    check(rep_from_rows(rows), configs.Clustering(eps=0.02, abs_k=-1, min_pts=2, use_hamming=True), tag="aa 250-384")
+    with pytest.raises(_lib.IdbError) as e:
+        engine.dbscan(rep_from_rows([(b"ACDEF" * 77, [0], [0])]), configs.C1)  # 385 letters, 5-letter alphabet
+    assert e.value.code == _lib.IDB_ERR_UNSUPPORTED and "384 letters" in str(e.value)
+
+
 # --------------------------------------------------------------------------
 # Edge cases and fallbacks
 def test_empty_and_single(gpu_ctx):
