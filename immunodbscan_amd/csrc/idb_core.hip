@@ -73,7 +73,7 @@ enum Slot {
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
     S_DIST, S_CNT2, S_EQG, S_RSCNT, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
-    S_COUNT
+    S_ELOW, S_COUNT
 };
 
 struct idb_ctx {
@@ -462,6 +462,13 @@ __global__ void k_emit_entries(const uint32_t *uniq, const uint32_t *num, const 
     }
 }
 
+// Word k of a gene bitset restricted to the genes below gene g.
+__device__ __forceinline__ uint32_t below_gene(uint32_t w, uint32_t k, uint32_t g)
+{
+    if (k * 32 >= g) return 0u;
+    return (k * 32 + 32 > g) ? (w & ((1u << (g - k * 32)) - 1u)) : w;
+}
+
 // Residues mod 16 of the set bits of `w` below gene `g` (all bits when g is
 // ~0u), folded into a 16-bit mask.
 __device__ __forceinline__ uint32_t gene_residues(const int32_t *w, uint32_t nw, uint32_t g)
@@ -483,7 +490,7 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
                                const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len, uint32_t lb,
                                uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag,
-                               uint32_t *eqg, uint32_t *pe, uint32_t *par_e, uint32_t *cmin_e)
+                               uint32_t *eqg, uint32_t *pe, uint32_t *par_e, uint32_t *cmin_e, uint4 *elow)
 {
     __shared__ uint8_t cm[256];
     __shared__ uint32_t qgc[256 * IDB_QGRAM_WORDS];  // one 64-byte 3-gram count row per thread
@@ -516,6 +523,15 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
             sj = gene_residues(jb + (size_t)u * jw, jw, bl % gj);
         }
         const uint32_t chk = single ? 1u : ((sv | sj) != 0u);
+        if (elow) {
+            // the row's V genes below the bucket's V gene, then its J genes
+            // below the bucket's J gene (canonical mode, vw + jw <= 4)
+            const uint32_t bl = (uint32_t)(bucket % gvj), gv = bl / gj, gjb = bl % gj;
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            for (uint32_t k = 0; k < vw; ++k) w[k] = below_gene((uint32_t)vb[(size_t)u * vw + k], k, gv);
+            for (uint32_t k = 0; k < jw; ++k) w[vw + k] = below_gene((uint32_t)jb[(size_t)u * jw + k], k, gjb);
+            elow[e] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
         r.len_chk = L | (chk << 16);
         r.t2 = (uint32_t)(uint16_t)tlut_c[L] | ((uint32_t)(uint16_t)tlut_f[L] << 16);
         r.sig = sv | (sj << 16);
@@ -534,7 +550,7 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
             }
         r.pres1 = p1;
         r.pres2 = p2;
-        r.pad = 0;
+        r.len = L;
         ent[e] = r;
         if (ecodes) {
             // letters past the row's end carry the sentinel code `pad`
@@ -1303,9 +1319,13 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         const bool qgram = !ham && !learned && K <= 4 && maxlen >= 16;
         uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
         uint32_t *par_e = buf<uint32_t>(c, S_PARE, E), *cmin_e = buf<uint32_t>(c, S_CMINE, E);
+        // The one-word count pass tests the canonical bucket on the entries'
+        // own genes below the bucket's (V words, then J words), when they fit
+        // four words.
+        uint4 *elow = (filt && !single && !learned && vw + jw <= 4) ? buf<uint4>(c, S_ELOW, E) : nullptr;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
-                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e);
+                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e, elow);
         ua.pe = pe;
         ua.par_e = par_e;
         ua.cmin_e = cmin_e;
@@ -1365,6 +1385,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         a.t_const = (p->abs_k >= 0 && !ham) ? (int32_t)tc[0] : -1;
         a.qgram_filter = qgram ? 1u : 0u;
         a.e_qg = eqg;
+        a.e_low = elow;
         a.cnt = cnt;
         a.work = work;
         // Neighbour pairs found by the count pass are kept for the fuse step;

@@ -89,10 +89,11 @@ struct __align__(16) idb_entry {
                                   // or every row in single-bucket mode)
     uint32_t t2;                  // threshold at this length: count pass (int16, low half),
                                   // fuse pass (int16, high half)
+    uint32_t len;                 // the length alone (the second half is the count pass's
+                                  // query record as it stands)
+    uint32_t pres1, pres2;        // letters present once / twice (absent-letter filter)
     uint32_t sig;                 // residues mod 16 of the genes the canonical test looks at:
                                   // V in the low half, J in the high half
-    uint32_t pres1, pres2;        // letters present once / twice (absent-letter filter)
-    uint32_t pad;
 };
 
 // Uniform parameters of a pair-kernel launch.
@@ -111,6 +112,7 @@ struct idb_pair_args {
     uint32_t band_filter;         // 1: absent-letter filter before Myers (alphabets <= 31 letters)
     uint32_t qgram_filter;        // 1: 3-gram profile filter before Myers (alphabets <= 4 letters)
     const uint32_t *e_qg;         // 3-gram profiles, IDB_QGRAM_WORDS per entry
+    const uint4 *e_low;           // per entry: its genes below the bucket's (V words, then J words), or null
     int32_t *cnt;                 // count pass output
     const int32_t *core;          // fuse pass input (per row)
     uint32_t *par;                // fuse pass: union-find parents

@@ -28,6 +28,7 @@
 // non-core row records its minimum-index core neighbour with atomicMin (the
 // canonical form of the reference's first-come atomicCAS at :113-122).
 #include <algorithm>
+#include <type_traits>
 
 #include "idb_internal.h"
 #include "pairdist.h"
@@ -239,11 +240,26 @@ __device__ __forceinline__ idb_entry load_entry(const idb_entry *e)
     r.cp = a.y;
     r.len_chk = a.z;
     r.t2 = a.w;
-    r.sig = b.x;
+    r.len = b.x;
     r.pres1 = b.y;
     r.pres2 = b.z;
-    r.pad = 0;
+    r.sig = b.w;
     return r;
+}
+
+// A tile record: the list is read-only during a pass and the index is
+// wave-uniform, so this is a scalar load (waited on by lgkmcnt, which keeps
+// it clear of the vector loads in flight).
+__device__ __forceinline__ idb_tile tile_rec(const idb_pair_args &a, uint32_t ti)
+{
+    typedef const __attribute__((address_space(4))) uint32_t *const_words;
+    const const_words w = (const_words)a.tiles + 4 * (size_t)ti;
+    idb_tile t;
+    t.q0 = w[0];
+    t.t0 = w[1];
+    t.bucket = w[2];
+    t.shape = w[3];
+    return t;
 }
 
 // What a Myers tile needs from global memory, loaded one tile ahead: the
@@ -252,20 +268,43 @@ __device__ __forceinline__ idb_entry load_entry(const idb_entry *e)
 template <bool CODES>
 struct TilePre {
     idb_tile tile;
-    idb_entry te, qe;
+    uint4 te0, te1, qe0, qe1;  // the two halves of the target's and the query's records
     uint4 qc0, qc1, tc0, tc1;
 };
+
+// A record from its two 16-byte halves (kept as loaded, so that the loads
+// write their registers directly).
+__device__ __forceinline__ idb_entry entry_of(const uint4 &p0, const uint4 &p1)
+{
+    idb_entry r;
+    r.row = p0.x;
+    r.cp = p0.y;
+    r.len_chk = p0.z;
+    r.t2 = p0.w;
+    r.len = p1.x;
+    r.pres1 = p1.y;
+    r.pres2 = p1.z;
+    r.sig = p1.w;
+    return r;
+}
 
 template <bool CODES>
 __device__ __forceinline__ void tile_load(const idb_pair_args &a, const idb_tile &t, uint32_t lane, TilePre<CODES> &d)
 {
     const uint32_t qn = t.shape & 0xFFu, tn = (t.shape >> 8) & 0xFFu;
     d.tile = t;
-    d.te = idb_entry{};
-    d.qe = idb_entry{};
+    d.te0 = d.te1 = d.qe0 = d.qe1 = make_uint4(0u, 0u, 0u, 0u);
     d.qc0 = d.qc1 = d.tc0 = d.tc1 = make_uint4(0u, 0u, 0u, 0u);
-    if (lane < tn) d.te = load_entry(a.ent + t.t0 + lane);
-    if (lane < qn) d.qe = load_entry(a.ent + t.q0 + lane);
+    if (lane < tn) {
+        const uint4 *p = (const uint4 *)(a.ent + t.t0 + lane);
+        d.te0 = p[0];
+        d.te1 = p[1];
+    }
+    if (lane < qn) {
+        const uint4 *p = (const uint4 *)(a.ent + t.q0 + lane);
+        d.qe0 = p[0];
+        d.qe1 = p[1];
+    }
     if (CODES) {
         const bool wide = a.S > 16;
         if (lane < qn) {
@@ -525,7 +564,8 @@ __device__ __forceinline__ void myers_batch1(const idb_pair_args &a, const uint3
                                              uint32_t n, uint32_t lane, const idb_entry &qe, const idb_entry &te,
                                              int32_t core_y, int32_t q_core, EdgeStage &est, int32_t *qacc,
                                              int32_t *tacc, unsigned long long &hits, unsigned long long &mlet,
-                                             const uint4 &qc0, const uint4 &qc1, bool wide)
+                                             const uint4 &qc0, const uint4 &qc1, bool wide,
+                                             const uint4 *qcodes = nullptr)
 {
     const bool act = lane < n;
     const uint32_t e = queue[lane < n ? lane : 0u];
@@ -541,17 +581,24 @@ __device__ __forceinline__ void myers_batch1(const idb_pair_args &a, const uint3
     const uint32_t qt2 = (uint32_t)__shfl((int)qe.t2, qi, 64);
     const int32_t core_x = __shfl(q_core, qi, 64);
     uint32_t w[8];
-    w[0] = (uint32_t)__shfl((int)qc0.x, qi, 64);
-    w[1] = (uint32_t)__shfl((int)qc0.y, qi, 64);
-    w[2] = (uint32_t)__shfl((int)qc0.z, qi, 64);
-    w[3] = (uint32_t)__shfl((int)qc0.w, qi, 64);
-    if (wide) {
-        w[4] = (uint32_t)__shfl((int)qc1.x, qi, 64);
-        w[5] = (uint32_t)__shfl((int)qc1.y, qi, 64);
-        w[6] = (uint32_t)__shfl((int)qc1.z, qi, 64);
-        w[7] = (uint32_t)__shfl((int)qc1.w, qi, 64);
+    if (qcodes) {
+        // the tile's query codes staged in LDS, two uint4 per query
+        const uint4 c0 = qcodes[2 * qi], c1 = wide ? qcodes[2 * qi + 1] : make_uint4(0u, 0u, 0u, 0u);
+        w[0] = c0.x, w[1] = c0.y, w[2] = c0.z, w[3] = c0.w;
+        w[4] = c1.x, w[5] = c1.y, w[6] = c1.z, w[7] = c1.w;
     } else {
-        w[4] = w[5] = w[6] = w[7] = 0u;
+        w[0] = (uint32_t)__shfl((int)qc0.x, qi, 64);
+        w[1] = (uint32_t)__shfl((int)qc0.y, qi, 64);
+        w[2] = (uint32_t)__shfl((int)qc0.z, qi, 64);
+        w[3] = (uint32_t)__shfl((int)qc0.w, qi, 64);
+        if (wide) {
+            w[4] = (uint32_t)__shfl((int)qc1.x, qi, 64);
+            w[5] = (uint32_t)__shfl((int)qc1.y, qi, 64);
+            w[6] = (uint32_t)__shfl((int)qc1.z, qi, 64);
+            w[7] = (uint32_t)__shfl((int)qc1.w, qi, 64);
+        } else {
+            w[4] = w[5] = w[6] = w[7] = 0u;
+        }
     }
     const uint32_t lx = act ? lxq : 0u;
     const int32_t t = !act ? -1 : (lx >= tm ? t_of(qt2, FUSE) : t_of(tt2, FUSE));
@@ -567,11 +614,17 @@ __device__ __forceinline__ void myers_batch1(const idb_pair_args &a, const uint3
         uint32_t eq[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) eq[b] = pcol[((word >> (8 * b)) & 0xFFu) * 64];
+        if (!__ballot(4u * k + 4u > lx)) {
+            // every lane's row covers the whole word: no per-letter select
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            idb_myers<1> nx = st;
-            idb_myers_step<1>(nx, eq + b, top_w, top_b);
-            if (4u * k + b < lx) st = nx;
+            for (int b = 0; b < 4; ++b) idb_myers_step<1>(st, eq + b, top_w, top_b);
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                idb_myers<1> nx = st;
+                idb_myers_step<1>(nx, eq + b, top_w, top_b);
+                if (4u * k + b < lx) st = nx;
+            }
         }
     }
     const int32_t dist = tm ? st.score : (int32_t)lx;
@@ -618,21 +671,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t T = a.n_tiles;
     uint32_t ti = wave0;
     uint32_t ti1 = step_tile(ti, total_waves, T);
-    idb_tile rec1 = ti1 < T ? a.tiles[ti1] : idb_tile{};
+    idb_tile rec1 = ti1 < T ? tile_rec(a, ti1) : idb_tile{};
     TilePre<FILT == 1> cur{};
-    if (ti < T) tile_load<FILT == 1>(a, a.tiles[ti], lane, cur);
+    if (ti < T) tile_load<FILT == 1>(a, tile_rec(a, ti), lane, cur);
     const bool wide = a.S > 16;
     while (ti < T) {
         TilePre<FILT == 1> nxt{};
         if (ti1 < T) tile_load<FILT == 1>(a, rec1, lane, nxt);
         const uint32_t ti2 = step_tile(ti1, total_waves, T);
-        const idb_tile rec2 = ti2 < T ? a.tiles[ti2] : idb_tile{};
+        const idb_tile rec2 = ti2 < T ? tile_rec(a, ti2) : idb_tile{};
         const idb_tile tile = cur.tile;
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
         const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
-        const idb_entry te = cur.te, qe = cur.qe;
+        const idb_entry te = entry_of(cur.te0, cur.te1), qe = entry_of(cur.qe0, cur.qe1);
         const uint4 qc0 = cur.qc0, qc1 = cur.qc1;
         const uint32_t y = te.row, m = te.len_chk & 0xFFFFu, cy = te.cp;
         const bool chk_y = (te.len_chk >> 16) & 1u;
@@ -943,12 +996,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __host__ __device__ inline size_t idb_count1_lds_per_wave(uint32_t K)
 {
     // Peq columns, edge staging, the Myers queue, count accumulators of
-    // queries and targets, the queries' 16-byte gate records
-    return ((size_t)K * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t) + IDB_TILE * sizeof(uint4);
+    // queries and targets, the queries' 16-byte gate records and their letter
+    // codes (32 bytes)
+    return ((size_t)K * 64 + 2 * IDB_TILE + IDB_TILE + 2 * IDB_TILE) * sizeof(uint32_t) + 3 * IDB_TILE * sizeof(uint4);
 }
 
 template <bool TCONST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_count1(idb_pair_args a, uint32_t wpb)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_count1(idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
@@ -960,29 +1014,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     int32_t *qacc = (int32_t *)(peq + K * 64 + 3 * IDB_TILE);
     int32_t *tacc = qacc + IDB_TILE;
     uint4 *gq = (uint4 *)(tacc + IDB_TILE);
+    uint4 *gqc = gq + IDB_TILE;  // query codes, two uint4 per query
     const uint32_t total_waves = gridDim.x * wpb;
     const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + wib * gridDim.x;
     const int32_t t_const = a.t_const;
+    const bool exact_low = a.e_low != nullptr;
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
+    unsigned long long glane = 0, gllane = 0;  // per-lane gated pairs and their target letters (TCONST)
 
     const uint32_t T = a.n_tiles;
     uint32_t ti = wave0;
     uint32_t ti1 = step_tile(ti, total_waves, T);
-    idb_tile rec1 = ti1 < T ? a.tiles[ti1] : idb_tile{};
+    idb_tile rec1 = ti1 < T ? tile_rec(a, ti1) : idb_tile{};
     TilePre<true> cur{};
-    if (ti < T) tile_load<true>(a, a.tiles[ti], lane, cur);
+    uint4 cur_tlow = zero4, cur_qlow = zero4;
+    // a tile's genes below the bucket's (exact gene test), loaded with it
+    auto low_load = [&](const idb_tile &t, uint4 &tl, uint4 &ql) {
+        tl = ql = zero4;
+        if (!exact_low) return;
+        if (lane < ((t.shape >> 8) & 0xFFu)) tl = a.e_low[t.t0 + lane];
+        if (lane < (t.shape & 0xFFu)) ql = a.e_low[t.q0 + lane];
+    };
+    if (ti < T) {
+        const idb_tile r0 = tile_rec(a, ti);
+        tile_load<true>(a, r0, lane, cur);
+        low_load(r0, cur_tlow, cur_qlow);
+    }
     const bool wide = a.S > 16;
+    // the previous tile's counts (row, amount) for queries and targets
+    int32_t p_qa = 0, p_ta = 0;
+    uint32_t p_qrow = 0, p_trow = 0;
+    auto flush_counts = [&]() {
+        if (p_qa) atomicAdd(a.cnt + p_qrow, p_qa);
+        if (p_ta) atomicAdd(a.cnt + p_trow, p_ta);
+        p_qa = p_ta = 0;
+    };
     while (ti < T) {
-        TilePre<true> nxt{};
-        if (ti1 < T) tile_load<true>(a, rec1, lane, nxt);
-        const uint32_t ti2 = step_tile(ti1, total_waves, T);
-        const idb_tile rec2 = ti2 < T ? a.tiles[ti2] : idb_tile{};
+        // The tile's loads (issued one tile ago) are waited for here, once,
+        // and the next tile's are issued only after everything of this tile
+        // is in registers or LDS: the wait counter is in order, so a first
+        // use after the next loads would wait for those too.  The count
+        // atomics of a tile go out after the next tile's loads, so that they
+        // do not hold up this wait.
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         const idb_tile tile = cur.tile;
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
-        const idb_entry te = cur.te, qe = cur.qe;
-        const uint32_t y = te.row, m = te.len_chk & 0xFFFFu;
+        const idb_entry te = entry_of(cur.te0, cur.te1), qe = entry_of(cur.qe0, cur.qe1);
+        const uint32_t y = te.row, m = te.len;  // every field of the record is used: no register of a load in flight is reused
         const bool chk_y = (te.len_chk >> 16) & 1u;
         const int32_t t_m = t_of(te.t2, false);
         // Peq of this lane's target
@@ -995,14 +1076,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                 if (i < m) atomicOr(peq + ((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) * 64 + lane, 1u << i);
         }
         // gate records: length | check flag << 16 | count threshold (int8) << 24,
-        // presence planes, gene residues
+        // presence planes, gene residues; the queries' codes and low genes
         if (lane < qn) {
-            const int32_t tq = t_of(qe.t2, false);
-            const uint32_t t8 = (uint32_t)(uint8_t)(int8_t)(tq > 127 ? 127 : tq);
-            gq[lane] = make_uint4((qe.len_chk & 0x1FFFFu) | (t8 << 24), qe.pres1, qe.pres2, qe.sig);
+            if (TCONST) {
+                gq[lane] = cur.qe1;  // length, presence planes, gene residues
+            } else {
+                const int32_t tq = t_of(qe.t2, false);
+                const uint32_t t8 = (uint32_t)(uint8_t)(int8_t)(tq > 127 ? 127 : tq);
+                gq[lane] = make_uint4((qe.len_chk & 0x1FFFFu) | (t8 << 24), qe.pres1, qe.pres2, qe.sig);
+            }
         }
+        gqc[2 * lane] = cur.qc0;
+        gqc[2 * lane + 1] = cur.qc1;
+        const uint4 tlow = cur_tlow, qlow = cur_qlow;
+        // with the exact test, a target takes part only if it has low genes
+        const bool chk_yl = chk_y && (!exact_low || (tlow.x | tlow.y | tlow.z | tlow.w) != 0u);
         qacc[lane] = 0;
         tacc[lane] = 0;
+        // the next tile's loads
+        TilePre<true> nxt{};
+        uint4 nxt_tlow = zero4, nxt_qlow = zero4;
+        if (ti1 < T) {
+            tile_load<true>(a, rec1, lane, nxt);
+            low_load(rec1, nxt_tlow, nxt_qlow);
+        }
+        const uint32_t ti2 = step_tile(ti1, total_waves, T);
+        const idb_tile rec2 = ti2 < T ? tile_rec(a, ti2) : idb_tile{};
+        flush_counts();
         const uint64_t tmask = __ballot(tvalid);
         const uint32_t y1 = te.pres1, y2 = te.pres2, ny1 = ~te.pres1;
         uint32_t tg = 0, tgl = 0, tmc = 0, qlen = 0;
@@ -1031,21 +1131,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             vm = lanes & __ballot(dl <= t);
             if (((qx >> 16) & 1u) && vm) {
                 // the query may share a lower gene with the target: the
-                // canonical-bucket test for targets that may too and whose
-                // residues overlap (exact test on the bitsets)
-                const bool need = __builtin_amdgcn_inverse_ballot_w64(vm) && chk_y && (q.w & te.sig) != 0u;
-                const uint64_t nm = __ballot(need);
-                if (nm) {
-                    const uint32_t x = __builtin_amdgcn_readlane(qe.row, qi);
-                    bool ok = true;
-                    if (need) ok = pair_in_bucket(a, x, y, tile.bucket);
-                    vm &= ~nm | __ballot(ok);
+                // canonical-bucket test for targets that may too
+                if (exact_low) {
+                    const uint32_t common = (__builtin_amdgcn_readlane(qlow.x, qi) & tlow.x) |
+                                            (__builtin_amdgcn_readlane(qlow.y, qi) & tlow.y) |
+                                            (__builtin_amdgcn_readlane(qlow.z, qi) & tlow.z) |
+                                            (__builtin_amdgcn_readlane(qlow.w, qi) & tlow.w);
+                    vm &= ~__ballot(common != 0u);
+                } else {
+                    // residues overlap: exact test on the bitsets
+                    const bool need = __builtin_amdgcn_inverse_ballot_w64(vm) && chk_y && (q.w & te.sig) != 0u;
+                    const uint64_t nm = __ballot(need);
+                    if (nm) {
+                        const uint32_t x = __builtin_amdgcn_readlane(qe.row, qi);
+                        bool ok = true;
+                        if (need) ok = pair_in_bucket(a, x, y, tile.bucket);
+                        vm &= ~nm | __ballot(ok);
+                    }
                 }
             }
             const uint32_t nv = (uint32_t)__popcll(vm);
             tg += nv;
             tgl += nv * lx;
             return vm & __ballot((int32_t)(lbx > lby ? lbx : lby) <= t);
+        };
+        // A full batch (the first IDB_TILE queue entries) is verified, and
+        // the entries past it move to the front of the queue.
+        auto full_batch = [&]() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            myers_batch1<false>(a, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters, zero4,
+                                zero4, wide, gqc);
+            const uint32_t rest = qlen - IDB_TILE;
+            const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            if (lane < rest) queue[lane] = moved;
+            qlen = rest;
         };
         auto enqueue = [&](uint32_t qi, uint64_t pm) {
             if (!pm) return;
@@ -1057,47 +1177,136 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             const uint32_t np = (uint32_t)__popcll(pm);
             qlen += np;
             tmc += np;
-            if (qlen >= IDB_TILE) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                myers_batch1<false>(a, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters,
-                                    cur.qc0, cur.qc1, wide);
-                const uint32_t rest = qlen - IDB_TILE;
-                const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                if (lane < rest) queue[lane] = moved;
-                qlen = rest;
-            }
+            if (qlen >= IDB_TILE) full_batch();
         };
-        // two queries per iteration, both records read before either is
-        // used (gq[qi + 1] is inside the 64 records even past qn)
-        for (uint32_t qi = 0; qi < qn; qi += 2) {
-            const uint4 q0 = gq[qi], q1 = gq[qi + 1];
-            uint64_t v0 = 0, v1 = 0;
-            const uint64_t p0 = test(qi, q0, v0);
-            const uint64_t p1 = qi + 1 < qn ? test(qi + 1, q1, v1) : 0ull;
-            enqueue(qi, p0);
-            enqueue(qi + 1, p1);
+        if (TCONST) {
+            // Constant threshold: the band and both letter bounds fold into
+            // one maximum per pair, with no branch per query.  Queries run in
+            // windows of up to 32, QG at a time; a pair that passes sets
+            // bit r - 1 - k of the lane's mask mb (query w0 + k, r queries in
+            // the window), and once per window the masks become Myers queue
+            // entries.  gq holds the bare lengths; records past qn are
+            // sentinels of length 0x7FFF, and targets past tn take length
+            // 0xFFFF, so both fail the band.  Gated pairs are counted per
+            // lane.  Queries that may share a lower gene with their targets
+            // (chkq) are corrected after each window by the exact gene test.
+            if (lane >= qn) gq[lane] = make_uint4(0x7FFFu, 0u, 0u, 0u);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            const uint64_t chkq = __ballot(lane < qn && ((qe.len_chk >> 16) & 1u));
+            const uint32_t mlane = tvalid ? m : 0xFFFFu;
+            const int32_t t = t_const;
+            uint32_t gcnt = 0;  // gated pairs of this lane's target in this tile
+            constexpr uint32_t QG = 8;
+            auto group = [&](auto diag_c, uint32_t q0, uint32_t &mb) {
+                constexpr bool DIAG = decltype(diag_c)::value;
+#pragma unroll
+                for (uint32_t k = 0; k < QG; ++k) {
+                    const uint4 q = gq[q0 + k];
+                    const uint32_t dl = __builtin_amdgcn_sad_u16(q.x, mlane, 0u);
+                    const uint32_t lbx = __builtin_popcount(q.y & ny1) + __builtin_popcount(q.z & ny1);
+                    const uint32_t lby = __builtin_popcount(y1 & ~q.y) + __builtin_popcount(y2 & ~q.y);
+                    const uint32_t mx = max(dl, max(lbx, lby));
+                    bool band = (int32_t)dl <= t, pass = (int32_t)mx <= t;
+                    if (DIAG) {
+                        band = band && lane > q0 + k;
+                        pass = pass && lane > q0 + k;
+                    }
+                    gcnt += band ? 1u : 0u;
+                    mb = mb + mb + (pass ? 1u : 0u);
+                }
+            };
+            for (uint32_t w0 = 0; w0 < qn; w0 += 32) {
+                const uint32_t r = min(32u, (qn - w0 + QG - 1u) & ~(QG - 1u));
+                uint32_t mb = 0;
+                for (uint32_t g = 0; g < r; g += QG) {
+                    if (diag) group(std::true_type{}, w0 + g, mb);
+                    else group(std::false_type{}, w0 + g, mb);
+                }
+                // exact gene test for the window's queries that need it:
+                // pairs it rejects leave the gated count and the masks
+                for (uint64_t cm = (chkq >> w0) & (r == 32 ? 0xFFFFFFFFull : ((1ull << r) - 1ull)); cm;
+                     cm &= cm - 1ull) {
+                    const uint32_t k = (uint32_t)__builtin_ctzll(cm), qi = w0 + k;
+                    const uint4 q = gq[qi];
+                    const uint32_t dl = __builtin_amdgcn_sad_u16(q.x, mlane, 0u);
+                    bool ok = true;
+                    if (exact_low) {
+                        // a common gene below the bucket's: not this pair's bucket
+                        const uint32_t common = (__builtin_amdgcn_readlane(qlow.x, qi) & tlow.x) |
+                                                (__builtin_amdgcn_readlane(qlow.y, qi) & tlow.y) |
+                                                (__builtin_amdgcn_readlane(qlow.z, qi) & tlow.z) |
+                                                (__builtin_amdgcn_readlane(qlow.w, qi) & tlow.w);
+                        ok = !((int32_t)dl <= t && (!diag || lane > qi) && chk_yl && common != 0u);
+                    } else {
+                        const bool need = (int32_t)dl <= t && (!diag || lane > qi) && chk_y && (q.w & te.sig) != 0u;
+                        if (!__ballot(need)) continue;
+                        const uint32_t x = __builtin_amdgcn_readlane(qe.row, qi);
+                        if (need) ok = pair_in_bucket(a, x, y, tile.bucket);
+                    }
+                    if (!ok) {
+                        --gcnt;
+                        mb &= ~(1u << (r - 1u - k));
+                    }
+                }
+                for (;;) {
+                    const uint64_t any = __ballot(mb != 0u);
+                    if (!any) break;
+                    // the earliest query still set in each lane
+                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(mb | 1u);
+                    if (mb != 0u) {
+                        const uint32_t below =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
+                        queue[qlen + below] = (uint16_t)(((w0 + r - 1u - hb) << 8) | lane);
+                        mb &= ~(1u << hb);
+                    }
+                    const uint32_t np = (uint32_t)__popcll(any);
+                    qlen += np;
+                    tmc += np;
+                    if (qlen >= IDB_TILE) full_batch();
+                }
+            }
+            glane += gcnt;
+            gllane += (unsigned long long)gcnt * m;
+        } else {
+            // two queries per iteration, both records read before either is
+            // used (gq[qi + 1] is inside the 64 records even past qn)
+            for (uint32_t qi = 0; qi < qn; qi += 2) {
+                const uint4 q0 = gq[qi], q1 = gq[qi + 1];
+                uint64_t v0 = 0, v1 = 0;
+                const uint64_t p0 = test(qi, q0, v0);
+                const uint64_t p1 = qi + 1 < qn ? test(qi + 1, q1, v1) : 0ull;
+                enqueue(qi, p0);
+                enqueue(qi + 1, p1);
+            }
         }
         if (qlen) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            myers_batch1<false>(a, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters, cur.qc0,
-                                cur.qc1, wide);
+            myers_batch1<false>(a, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters, zero4, zero4,
+                                wide, gqc);
         }
         gated += tg;
         gletters += tgl;
         mpairs += tmc;
-        // one atomic per row and tile
+        // one atomic per row and tile, issued during the next tile
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        const int32_t qa = qacc[lane], ta = tacc[lane];
-        if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
-        if (tvalid && ta) atomicAdd(a.cnt + y, ta);
+        p_qa = lane < qn ? qacc[lane] : 0;
+        p_ta = tvalid ? tacc[lane] : 0;
+        p_qrow = qe.row;
+        p_trow = y;
         cur = nxt;
+        cur_tlow = nxt_tlow;
+        cur_qlow = nxt_qlow;
         ti = ti1;
         ti1 = ti2;
         rec1 = rec2;
     }
+    flush_counts();
     finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
+    if (TCONST) {
+        gated = wave_sum_u64(glane);
+        gletters = wave_sum_u64(gllane);
+    }
     if (lane == 0 && a.work) {
         atomicAdd(work_copy(a) + 0, gated);
         atomicAdd(work_copy(a) + 1, hits);
@@ -1146,7 +1355,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
     unsigned long long gated = 0, hits = 0, gletters = 0, mletters = 0, mpairs = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
-        const idb_tile tile = a.tiles[ti];
+        const idb_tile tile = tile_rec(a, ti);
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
@@ -1357,7 +1566,7 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     unsigned long long gated = 0, hits = 0, gletters = 0;
 
     for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
-        const idb_tile tile = a.tiles[ti];
+        const idb_tile tile = tile_rec(a, ti);
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
         const bool tvalid = lane < tn;
@@ -1755,10 +1964,14 @@ hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
     // one resident round of 4-wave blocks in XCD order) and on 20M
     // nucleotide rows (1.10 against 1.31 s).
     idb_pair_args a = a_in;
+    const bool count1 = W == 1 && a.band_filter && !FUSE && !a.single_bucket;
     if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
-        a.tune_wpb = 1;
-        a.tune_grid_mul = W == 1 ? 4u : 8u;
-        a.no_xcd_order = 1;
+        // the one-word count pass (k_count1): 4-wave blocks in XCD order,
+        // twice the resident grid (the autotuner's pick on 200k protein
+        // rows: 0.230 against 0.263 ms for the shape above)
+        a.tune_wpb = count1 ? 4u : 1u;
+        a.tune_grid_mul = count1 ? 2u : (W == 1 ? 4u : 8u);
+        a.no_xcd_order = count1 ? 0u : 1u;
     }
     const size_t per_wave = idb_myers_lds_per_wave(a.K, W) + (a.qgram_filter ? idb_qgram_lds_per_wave() : 0);
     const size_t limit = idb_lds_limit();
@@ -1771,7 +1984,7 @@ hipError_t launch_myers(const idb_pair_args &a_in, uint32_t W, hipStream_t s)
     const size_t shm = per_wave * wpb;
     if (W == 1 && a.band_filter) {
         if (a.S > 32 || a.K > 32) return hipErrorInvalidValue;  // one-word rows, alphabets of at most 31 letters
-        if (!FUSE && !a.single_bucket) return launch_count1(a, s);
+        if (count1) return launch_count1(a, s);
         return launch_myers_w<1, FUSE, 1>(a, block, shm, wpb, s);
     }
     if (a.qgram_filter) return launch_myers_f<FUSE, 2>(a, W, block, shm, wpb, s);
