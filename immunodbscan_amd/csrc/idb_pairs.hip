@@ -665,6 +665,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // different blocks and so to different CUs, not to the waves of one block
     const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) + wib * gridDim.x;
     unsigned long long gated = 0, hits = 0, gletters = 0, mpairs = 0, mletters = 0;
+    unsigned long long glane = 0, gllane = 0;  // per-lane gated pairs and target letters (constant threshold)
 
     // static order; the next tile's data and the record of the one after are
     // loaded while the current tile runs
@@ -676,10 +677,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (ti < T) tile_load<FILT == 1>(a, tile_rec(a, ti), lane, cur);
     const bool wide = a.S > 16;
     while (ti < T) {
-        TilePre<FILT == 1> nxt{};
-        if (ti1 < T) tile_load<FILT == 1>(a, rec1, lane, nxt);
-        const uint32_t ti2 = step_tile(ti1, total_waves, T);
-        const idb_tile rec2 = ti2 < T ? tile_rec(a, ti2) : idb_tile{};
+        // The tile's records (loaded one tile ago) are waited for once; its
+        // other loads go out together below, and the next tile's records
+        // only once this tile's loaded data is in registers or LDS (the wait
+        // counter is in order: a first use after later loads waits for them).
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         const idb_tile tile = cur.tile;
         const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
         const bool diag = (tile.shape >> 16) & 1u;
@@ -709,17 +711,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     if (i < m) atomicOr(peq + ((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) * 64 + lane, 1u << i);
             }
         } else if (tvalid) {
-            // code words 16 bytes at a time (S is a multiple of 16)
+            // code words 16 bytes at a time (S is a multiple of 16), five
+            // loads in flight at once
             const uint4 *codes = (const uint4 *)(a.e_codes + (size_t)p * a.S);
-            for (uint32_t i0 = 0; i0 < m; i0 += 16) {
-                const uint4 v = codes[i0 >> 4];
-                const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+            constexpr uint32_t CG = 5;
+            for (uint32_t i0 = 0; i0 < m; i0 += 16 * CG) {
+                uint4 v[CG];
 #pragma unroll
-                for (uint32_t k = 0; k < 16; ++k) {
-                    const uint32_t i = i0 + k;
-                    if (i < m) {
-                        const uint32_t c = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-                        atomicOr(peq + (c * W + (i >> 5)) * 64 + lane, 1u << (i & 31));
+                for (uint32_t g = 0; g < CG; ++g) v[g] = i0 + 16 * g < m ? codes[(i0 >> 4) + g] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                for (uint32_t g = 0; g < CG; ++g) {
+                    const uint32_t ws[4] = {v[g].x, v[g].y, v[g].z, v[g].w};
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k) {
+                        const uint32_t i = i0 + 16 * g + k;
+                        if (i < m) {
+                            const uint32_t c = (ws[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+                            atomicOr(peq + (c * W + (i >> 5)) * 64 + lane, 1u << (i & 31));
+                        }
                     }
                 }
             }
@@ -740,6 +749,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         // query records for the loop: one LDS broadcast per query instead of
         // a chain of lane reads into scalar registers
         qrec[lane] = qe;
+        // genes below the bucket's (exact gene test), when the entries have them
+        const bool exact_low = a.e_low != nullptr && !a.single_bucket;
+        uint4 tlow = make_uint4(0u, 0u, 0u, 0u), qlow = tlow;
+        if (exact_low) {
+            if (tvalid) tlow = a.e_low[tile.t0 + lane];
+            if (lane < qn) qlow = a.e_low[tile.q0 + lane];
+        }
         uint32_t tq[FILT == 2 ? IDB_QGRAM_WORDS : 1];
         if (FILT == 2) {
             const uint4 *src = (const uint4 *)(a.e_qg + (size_t)p * IDB_QGRAM_WORDS);
@@ -759,6 +775,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         }
+        // (used here, so that they are waited for before the next loads)
+        const bool tlow_any = (tlow.x | tlow.y | tlow.z | tlow.w) != 0u;
+        const bool qlow_set = __ballot((qlow.x | qlow.y | qlow.z | qlow.w) != 0u) != 0ull;
+        // the next tile's records
+        TilePre<FILT == 1> nxt{};
+        if (ti1 < T) tile_load<FILT == 1>(a, rec1, lane, nxt);
+        const uint32_t ti2 = step_tile(ti1, total_waves, T);
+        const idb_tile rec2 = ti2 < T ? tile_rec(a, ti2) : idb_tile{};
 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         if (FILT == 1 && !FUSE) {
@@ -818,7 +842,124 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 }
             }
         }
-        for (uint32_t qi = (FILT == 1 && !FUSE) ? qn : 0u; qi < qn; ++qi) {
+        // Constant threshold over long rows with the 3-gram bound (the
+        // count pass of the nucleotide workloads): the band and the bound are
+        // tested with no branch per query, as in k_count1.  Queries run in
+        // windows of up to 32, four at a time; the second half of the
+        // profiles is read only when some pair of the four passes the first
+        // half.  A pair that passes sets bit r - 1 - k of the lane's mask mb
+        // (query w0 + k, r queries in the window); the exact gene test
+        // corrects each window; the bits then become Myers queue entries.
+        // Gated pairs are counted per lane (letters: the target's).
+        const bool fast2 = FILT == 2 && !FUSE && a.t_const >= 0 && !a.single_bucket;
+        if (FILT == 2 && !FUSE && fast2) {
+            const int32_t t = a.t_const;
+            const uint32_t mlane = tvalid ? m : 0xFFFFu;
+            const uint64_t chkq = __ballot(lane < qn && ((qe.len_chk >> 16) & 1u));
+            uint32_t gcnt = 0;
+            constexpr uint32_t QG = 4, HALF = IDB_QGRAM_WORDS / 8;  // uint4 reads per half profile
+            for (uint32_t w0 = 0; w0 < qn; w0 += 32) {
+                const uint32_t r = min(32u, (qn - w0 + QG - 1u) & ~(QG - 1u));
+                uint32_t mb = 0;
+                for (uint32_t g = 0; g < r; g += QG) {
+                    uint32_t dl[QG], l1[QG];
+#pragma unroll
+                    for (uint32_t k = 0; k < QG; ++k) {
+                        const uint32_t qi = w0 + g + k;
+                        const uint32_t ql = qi < qn ? qrec[qi].len : 0x7FFFu;  // past qn: fails the band
+                        dl[k] = __builtin_amdgcn_sad_u16(ql, mlane, 0u);
+                        const uint4 *qv = (const uint4 *)(qg + qi * IDB_QGRAM_WORDS);
+                        uint32_t l = 0;
+#pragma unroll
+                        for (uint32_t h = 0; h < HALF; ++h) {
+                            const uint4 v = qv[h];
+                            l = __builtin_amdgcn_sad_u8(v.x, tq[4 * h], l);
+                            l = __builtin_amdgcn_sad_u8(v.y, tq[4 * h + 1], l);
+                            l = __builtin_amdgcn_sad_u8(v.z, tq[4 * h + 2], l);
+                            l = __builtin_amdgcn_sad_u8(v.w, tq[4 * h + 3], l);
+                        }
+                        l1[k] = l;
+                    }
+                    bool any = false;
+#pragma unroll
+                    for (uint32_t k = 0; k < QG; ++k)
+                        any = any || ((int32_t)dl[k] <= t && (int32_t)l1[k] <= 6 * t && w0 + g + k < dlim);
+                    if (__ballot(any)) {
+#pragma unroll
+                        for (uint32_t k = 0; k < QG; ++k) {
+                            const uint4 *qv = (const uint4 *)(qg + (w0 + g + k) * IDB_QGRAM_WORDS);
+                            uint32_t l = l1[k];
+#pragma unroll
+                            for (uint32_t h = HALF; h < 2 * HALF; ++h) {
+                                const uint4 v = qv[h];
+                                l = __builtin_amdgcn_sad_u8(v.x, tq[4 * h], l);
+                                l = __builtin_amdgcn_sad_u8(v.y, tq[4 * h + 1], l);
+                                l = __builtin_amdgcn_sad_u8(v.z, tq[4 * h + 2], l);
+                                l = __builtin_amdgcn_sad_u8(v.w, tq[4 * h + 3], l);
+                            }
+                            l1[k] = l;
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t k = 0; k < QG; ++k) {
+                        const bool band = (int32_t)dl[k] <= t && w0 + g + k < dlim;
+                        gcnt += band ? 1u : 0u;
+                        mb = mb + mb + ((band && (int32_t)l1[k] <= 6 * t) ? 1u : 0u);
+                    }
+                }
+                // exact gene test for the window's queries that need it
+                for (uint64_t cm = (chkq >> w0) & (r == 32 ? 0xFFFFFFFFull : ((1ull << r) - 1ull)); cm;
+                     cm &= cm - 1ull) {
+                    const uint32_t k = (uint32_t)__builtin_ctzll(cm), qi = w0 + k;
+                    const idb_entry q = qrec[qi];
+                    const bool band = (int32_t)__builtin_amdgcn_sad_u16(q.len, mlane, 0u) <= t && qi < dlim;
+                    bool ok = true;
+                    if (exact_low) {
+                        const uint32_t common = (__builtin_amdgcn_readlane(qlow.x, qi) & tlow.x) |
+                                                (__builtin_amdgcn_readlane(qlow.y, qi) & tlow.y) |
+                                                (__builtin_amdgcn_readlane(qlow.z, qi) & tlow.z) |
+                                                (__builtin_amdgcn_readlane(qlow.w, qi) & tlow.w);
+                        ok = !(band && tlow_any && common != 0u);
+                    } else {
+                        const bool need = band && chk_y && (__builtin_amdgcn_readfirstlane(q.sig) & te.sig) != 0u;
+                        if (!__ballot(need)) continue;
+                        if (need) ok = pair_in_bucket(a, q.row, y, tile.bucket);
+                    }
+                    if (!ok) {
+                        --gcnt;
+                        mb &= ~(1u << (r - 1u - k));
+                    }
+                }
+                for (;;) {
+                    const uint64_t anyb = __ballot(mb != 0u);
+                    if (!anyb) break;
+                    // the earliest query still set in each lane
+                    const uint32_t hb = 31u - (uint32_t)__builtin_clz(mb | 1u);
+                    if (mb != 0u) {
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(anyb >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)anyb, 0u));
+                        queue[qlen + below] = (uint16_t)(((w0 + r - 1u - hb) << 8) | lane);
+                        mb &= ~(1u << hb);
+                    }
+                    const uint32_t np = (uint32_t)__popcll(anyb);
+                    qlen += np;
+                    tm += np;
+                    if (qlen >= IDB_TILE) {
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        myers_batch_any<W, FUSE>(a, tile, peq, queue, IDB_TILE, lane, qe, te, core_y, q_core, est,
+                                                 qacc, tacc, hits, mletters);
+                        const uint32_t rest = qlen - IDB_TILE;
+                        const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        if (lane < rest) queue[lane] = moved;
+                        qlen = rest;
+                    }
+                }
+            }
+            glane += gcnt;
+            gllane += (unsigned long long)gcnt * m;
+        }
+        for (uint32_t qi = ((FILT == 1 && !FUSE) || fast2) ? qn : 0u; qi < qn; ++qi) {
             const idb_entry q = qrec[qi];
             const uint32_t x = q.row;
             // the query's length and thresholds are wave-uniform: scalar registers
@@ -837,7 +978,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
             // V/J gate: only a query that may share a lower gene (or every
             // query in single-bucket mode) needs it, a wave-uniform test
-            if (a.single_bucket || chk_x) {
+            if (exact_low && chk_x) {
+                // a common gene below the bucket's: not this pair's bucket
+                if (qlow_set) {
+                    const uint32_t common = (__builtin_amdgcn_readlane(qlow.x, qi) & tlow.x) |
+                                            (__builtin_amdgcn_readlane(qlow.y, qi) & tlow.y) |
+                                            (__builtin_amdgcn_readlane(qlow.z, qi) & tlow.z) |
+                                            (__builtin_amdgcn_readlane(qlow.w, qi) & tlow.w);
+                    valid = valid && !(tlow_any && common != 0u);
+                }
+            } else if (a.single_bucket || chk_x) {
                 const uint32_t c = __builtin_amdgcn_readfirstlane(q.sig) & te.sig;
                 bool need;
                 if (a.single_bucket) {
@@ -969,6 +1119,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     if (!FUSE) finish_edges(a, est, lane);
     mletters = wave_sum_u64(mletters);
+    if (FILT == 2 && !FUSE) {
+        // the constant-threshold loop's per-lane counts
+        gated += wave_sum_u64(glane);
+        gletters += wave_sum_u64(gllane);
+    }
     if (lane == 0 && a.work) {
         atomicAdd(work_copy(a) + (FUSE ? 2 : 0), gated);
         if (!FUSE) {
