@@ -50,6 +50,8 @@ EXPORTED = [
     "idb_pair_distances",
     "idb_neighbour_pairs",
     "idb_encoder_create",
+    "idb_encoder_create_positions",
+    "idb_encoder_positions",
     "idb_encoder_destroy",
     "idb_encoder_last_error",
     "idb_encoder_param_count",
@@ -198,6 +200,8 @@ def _load() -> C.CDLL:
         "idb_pair_distances": ([vp, DS, vp, u64, u32, vp], C.c_int),
         "idb_neighbour_pairs": ([vp, DS, PR, vp, u64, C.POINTER(u64)], C.c_int),
         "idb_encoder_create": ([C.c_int, vp, u64, C.POINTER(vp)], C.c_int),
+        "idb_encoder_create_positions": ([C.c_int, vp, u32, u64, C.POINTER(vp)], C.c_int),
+        "idb_encoder_positions": ([vp], u32),
         "idb_encoder_destroy": ([vp], None),
         "idb_encoder_last_error": ([vp], C.c_char_p),
         "idb_encoder_param_count": ([], u32),

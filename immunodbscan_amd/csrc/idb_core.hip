@@ -1218,8 +1218,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
                                              std::to_string(maxlen) + " letters exceeds the LDS budget");
         S = std::max(16u, (maxlen + 15) / 16 * 16);
     }
-    if (learned && (ham || W != 1 || maxlen > IDB_ENC_P))
-        fail(c, IDB_ERR_UNSUPPORTED, "the learned filter needs Levenshtein mode and rows of at most 32 letters");
+    if (learned && (ham || maxlen > idb_encoder_positions(x->enc) || W > 5))
+        fail(c, IDB_ERR_UNSUPPORTED,
+             "the learned filter needs Levenshtein mode and rows no longer than the encoder's positions (" +
+                 std::to_string(idb_encoder_positions(x->enc)) + ")");
     if (learned && idb_encoder_device(x->enc) != c->device)
         fail(c, IDB_ERR_ARG, "the encoder lives on another device");
     {

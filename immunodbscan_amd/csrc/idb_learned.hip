@@ -6,16 +6,19 @@
 // of a tile whose dot product z_x . z_y reaches a threshold, and the exact
 // Myers kernel verifies every proposal, so distances are never approximated.
 //
-//   tokens (<= 32 letters) -> embedding 32 x 32 -> 3 x [conv1d k=5 'same',
+//   tokens (<= P letters) -> embedding 32 x 32 -> 3 x [conv1d k=5 'same',
 //   128 channels, GELU] -> mean over letters -> linear 128 -> 64 -> L2 norm.
+//
+// P, the positions per row, is fixed when the encoder is made: 32 for
+// protein rows, 160 for nucleotide rows of up to 160 letters.
 //
 // Weights are bfloat16 copies of float32 master weights (Adam keeps the
 // master copy); activations are bfloat16, all products accumulate in float32.
 // The three convolutions, their data gradients and their weight gradients run
 // on one MFMA GEMM kernel (v_mfma_f32_32x32x16_bf16) whose operand loaders
 // gather im2col windows straight from the activations, so no im2col matrix is
-// ever materialised.  Row m of an activation matrix is position m % 32 of
-// sequence m / 32; positions at or past the sequence's length are zero after
+// ever materialised.  Row m of an activation matrix is position m % P of
+// sequence m / P; positions at or past the sequence's length are zero after
 // every layer (each row is convolved with its own zero padding).
 #include <algorithm>
 #include <cmath>
@@ -32,7 +35,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t P = IDB_ENC_P, T = 5, NTOK = 32, C0 = 32, C = 128, D = 64;
+constexpr uint32_t T = 5, NTOK = 32, C0 = 32, C = 128, D = 64;
 constexpr float GELU_K = 0.7978845608028654f;  // sqrt(2 / pi)
 
 // Flat parameter layout (float32 master and bfloat16 copy share it).
@@ -180,8 +183,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm(LA la, LB lb, EP ep, uint32_t M
 __device__ __forceinline__ uint32_t tile_row(int r, uint32_t lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
 // im2col window of an activation matrix X[rows][cin]: element (m, t * cin +
-// c) is X[32 s + q][c] with q = p + t - 2 (forward) or q = p - t + 2 (data
-// gradient), zero outside the 32 positions of sequence s = m / 32.
+// c) is X[PP s + q][c] with q = p + t - 2 (forward) or q = p - t + 2 (data
+// gradient), zero outside the PP positions of sequence s = m / PP.
+template <uint32_t PP>
 struct LdConv {
     static constexpr bool COL = false;
     const uint16_t *X;
@@ -191,10 +195,10 @@ struct LdConv {
     {
         if (m >= M) return make_uint4(0, 0, 0, 0);
         const uint32_t sh = 31u - __builtin_clz(cin);
-        const int t = (int)(k >> sh), c = (int)(k & (cin - 1u)), p = (int)(m & 31u);
+        const int t = (int)(k >> sh), c = (int)(k & (cin - 1u)), p = (int)(m % PP);
         const int q = flip ? p - t + 2 : p + t - 2;
-        if (q < 0 || q >= (int)P) return make_uint4(0, 0, 0, 0);
-        return *(const uint4 *)(X + ((size_t)(m & ~31u) + q) * cin + c);
+        if (q < 0 || q >= (int)PP) return make_uint4(0, 0, 0, 0);
+        return *(const uint4 *)(X + ((size_t)(m - (uint32_t)p) + q) * cin + c);
     }
 };
 
@@ -224,7 +228,8 @@ struct LdGradT {
 };
 
 // Weight gradient, operand B: rows are im2col columns k' = t * cin + c, k runs
-// over activation rows m: X[32 s + p + t - 2][c] (8 consecutive c per load).
+// over activation rows m: X[PP s + p + t - 2][c] (8 consecutive c per load).
+template <uint32_t PP>
 struct LdConvT {
     static constexpr bool COL = true;
     const uint16_t *X;
@@ -233,16 +238,19 @@ struct LdConvT {
     {
         if (m >= M || kc >= T * cin) return make_uint4(0, 0, 0, 0);
         const uint32_t sh = 31u - __builtin_clz(cin);
-        const int t = (int)(kc >> sh), c = (int)(kc & (cin - 1u)), q = (int)(m & 31u) + t - 2;
-        if (q < 0 || q >= (int)P) return make_uint4(0, 0, 0, 0);
-        return *(const uint4 *)(X + ((size_t)(m & ~31u) + q) * cin + c);
+        const uint32_t p = m % PP;
+        const int t = (int)(kc >> sh), c = (int)(kc & (cin - 1u)), q = (int)p + t - 2;
+        if (q < 0 || q >= (int)PP) return make_uint4(0, 0, 0, 0);
+        return *(const uint4 *)(X + ((size_t)(m - p) + q) * cin + c);
     }
 };
 
 // Epilogues get one 32 x 32 result tile per call: rows row0 .. row0 + 31
-// (one sequence: row0 is a multiple of 32) and this lane's column `col`.
+// (within one sequence: row0 and PP are multiples of 32) and this lane's
+// column `col`.
 
 // Forward epilogue: bias, saved pre-activation, GELU, length mask.
+template <uint32_t PP>
 struct EpConvFwd {
     const uint16_t *bias;
     const uint32_t *len;
@@ -251,20 +259,21 @@ struct EpConvFwd {
     {
         if (col >= N || row0 >= M) return;
         const float b = bf2f(bias[col]);
-        const uint32_t L = len[row0 >> 5];
+        const uint32_t L = len[row0 / PP], p0 = row0 % PP;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const uint32_t p = tile_row(r, lane);
             const size_t at = (size_t)(row0 + p) * C + col;
             const float x = v[r] + b;
             if (H) H[at] = f2bf(x);
-            A[at] = p < L ? f2bf(gelu(x)) : (uint16_t)0;
+            A[at] = p0 + p < L ? f2bf(gelu(x)) : (uint16_t)0;
         }
     }
 };
 
 // Data-gradient epilogue: masked, times GELU' of the previous layer's saved
 // pre-activation (bfloat16 out), or the embedding gradient (float out).
+template <uint32_t PP>
 struct EpConvBwd {
     const uint16_t *Hprev;
     const uint32_t *len;
@@ -274,12 +283,12 @@ struct EpConvBwd {
     __device__ void tile(const f32x16 &v, uint32_t row0, uint32_t col, uint32_t lane, uint32_t M, uint32_t N) const
     {
         if (col >= N || row0 >= M) return;
-        const uint32_t L = len[row0 >> 5];
+        const uint32_t L = len[row0 / PP], p0 = row0 % PP;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const uint32_t p = tile_row(r, lane);
             const size_t at = (size_t)(row0 + p) * cin + col;
-            const bool valid = p < L;
+            const bool valid = p0 + p < L;
             if (dHprev) dHprev[at] = valid ? f2bf(v[r] * gelu_grad(bf2f(Hprev[at]))) : (uint16_t)0;
             else dX0[at] = valid ? v[r] : 0.f;
         }
@@ -303,9 +312,9 @@ struct EpAtomic {
 // ---------------------------------------------------------------------------
 // VALU kernels
 
-// Tokens of every row: vocabulary map, first 32 letters, token 0 past the end.
+// Tokens of every row: vocabulary map, first P letters, token 0 past the end.
 __global__ void k_tokenize(const uint8_t *seq, const uint64_t *off, const uint32_t *len, uint32_t n,
-                           const uint8_t *vocab, uint8_t *tok, uint32_t *tlen)
+                           const uint8_t *vocab, uint32_t P, uint8_t *tok, uint32_t *tlen)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t L = min(len[i], P);
@@ -317,7 +326,7 @@ __global__ void k_tokenize(const uint8_t *seq, const uint64_t *off, const uint32
 
 // Batch assembly: sequence b of the batch is row rows[b] (IDB_NONE: empty).
 __global__ void k_gather_rows(const uint8_t *tok, const uint32_t *tlen, const uint32_t *rows, uint32_t nb,
-                              uint8_t *btok, uint32_t *blen)
+                              uint32_t P, uint8_t *btok, uint32_t *blen)
 {
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         const uint32_t r = rows[b];
@@ -352,12 +361,13 @@ __global__ void k_pair_rows(const uint32_t *pairs, const uint32_t *order, uint64
     }
 }
 
-__global__ void k_embed(const uint8_t *btok, const uint32_t *blen, uint32_t nseq, const uint16_t *emb, uint16_t *X0)
+__global__ void k_embed(const uint8_t *btok, const uint32_t *blen, uint32_t nseq, uint32_t P, const uint16_t *emb,
+                        uint16_t *X0)
 {
     const uint32_t total = nseq * P * (C0 / 8);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         const uint32_t m = i / (C0 / 8), c8 = (i % (C0 / 8)) * 8;
-        const bool valid = (m & 31u) < blen[m >> 5];
+        const bool valid = (m % P) < blen[m / P];
         uint4 v = make_uint4(0, 0, 0, 0);
         if (valid) v = *(const uint4 *)(emb + (size_t)btok[m] * C0 + c8);
         *(uint4 *)(X0 + (size_t)m * C0 + c8) = v;
@@ -366,7 +376,7 @@ __global__ void k_embed(const uint8_t *btok, const uint32_t *blen, uint32_t nseq
 
 // Head: mean over the sequence's letters, projection, L2 norm.  One wave per
 // sequence: lanes hold channels l and l + 64, then output l.
-__global__ void k_head_fwd(const uint16_t *A3, const uint32_t *blen, uint32_t nseq, const uint16_t *wp_b,
+__global__ void k_head_fwd(const uint16_t *A3, const uint32_t *blen, uint32_t nseq, uint32_t P, const uint16_t *wp_b,
                            const uint16_t *bp, float *pool, float *u, float *z, uint16_t *zb)
 {
     __shared__ float sp[4][C];
@@ -430,7 +440,7 @@ __global__ void k_pair_loss(const float *z, const float *y, uint32_t npairs, flo
 // Head backward, one wave per sequence: du, dpool, and the last layer's
 // pre-activation gradient dH3 = (dpool / L) * GELU'(H3) at the valid letters.
 __global__ void k_head_bwd(const float *dz, const float *z, const float *u, const uint32_t *blen, uint32_t nseq,
-                           const uint16_t *wp_b, const uint16_t *H3, float *du_out, uint16_t *dH3)
+                           uint32_t P, const uint16_t *wp_b, const uint16_t *H3, float *du_out, uint16_t *dH3)
 {
     __shared__ float sdu[4][D];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -490,14 +500,15 @@ __global__ void k_bias_grad(const uint16_t *dH, uint32_t M, float *gb)
 }
 
 // Embedding gradient: scatter-add of dX0 rows by token, through LDS.
-__global__ void k_emb_grad(const float *dX0, const uint8_t *btok, const uint32_t *blen, uint32_t M, float *gemb)
+__global__ void k_emb_grad(const float *dX0, const uint8_t *btok, const uint32_t *blen, uint32_t M, uint32_t P,
+                           float *gemb)
 {
     __shared__ float acc[NTOK * C0];
     for (uint32_t i = threadIdx.x; i < NTOK * C0; i += blockDim.x) acc[i] = 0.f;
     __syncthreads();
     const uint32_t c = threadIdx.x & (C0 - 1);
     for (uint32_t m = blockIdx.x * (blockDim.x / C0) + threadIdx.x / C0; m < M; m += gridDim.x * (blockDim.x / C0)) {
-        if ((m & 31u) >= blen[m >> 5]) continue;
+        if ((m % P) >= blen[m / P]) continue;
         atomicAdd(&acc[(uint32_t)btok[m] * C0 + c], dX0[(size_t)m * C0 + c]);
     }
     __syncthreads();
@@ -559,6 +570,7 @@ void hcheck(hipError_t e, const char *what)
 // largest batch seen so far.
 struct idb_encoder {
     int device = 0;
+    uint32_t P = IDB_ENC_P;  // positions per row: 32 or 160
     hipStream_t stream = nullptr;
     uint8_t vocab[256];
     float *theta = nullptr, *grad = nullptr, *mom1 = nullptr, *mom2 = nullptr;
@@ -605,7 +617,7 @@ void ensure_workspace(idb_encoder *e, uint32_t nseq, bool train)
 {
     if (nseq <= e->cap_seq && (!train || e->cap_train)) return;
     free_workspace(e);
-    const size_t M = (size_t)nseq * P;
+    const size_t M = (size_t)nseq * e->P;
     e->btok = dalloc<uint8_t>(e->ws, M);
     e->blen = dalloc<uint32_t>(e->ws, nseq);
     e->brows = dalloc<uint32_t>(e->ws, nseq);
@@ -643,37 +655,45 @@ void refresh_transposed(idb_encoder *e)
 }
 
 // Forward over the batch in e->btok / e->blen (nseq, a multiple of 4).
-void forward(idb_encoder *e, uint32_t nseq, bool train, uint16_t *zb_out)
+template <uint32_t PP>
+void forward_p(idb_encoder *e, uint32_t nseq, bool train, uint16_t *zb_out)
 {
     hipStream_t s = e->stream;
-    const uint32_t M = nseq * P;
-    hipLaunchKernelGGL(k_embed, dim3(nblocks((size_t)M * C0 / 8)), dim3(256), 0, s, e->btok, e->blen, nseq,
+    const uint32_t M = nseq * PP;
+    hipLaunchKernelGGL(k_embed, dim3(nblocks((size_t)M * C0 / 8)), dim3(256), 0, s, e->btok, e->blen, nseq, PP,
                        e->theta_b + O_EMB, e->X0);
     const uint16_t *in = e->X0;
     uint32_t cin = C0;
     const uint32_t offs[3] = {O_W1, O_W2, O_W3}, boff[3] = {O_B1, O_B2, O_B3};
     for (int l = 0; l < 3; ++l) {
         uint16_t *out = train ? e->A[l] : e->A[l & 1];
-        LdConv la{in, cin, M, 0};
+        LdConv<PP> la{in, cin, M, 0};
         LdRows lb{e->theta_b + offs[l], T * cin, C};
-        EpConvFwd ep{e->theta_b + boff[l], e->blen, train ? e->H[l] : nullptr, out};
-        hipLaunchKernelGGL((k_gemm<LdConv, LdRows, EpConvFwd>), dim3((M + GM - 1) / GM, 1, 1), dim3(256), 0, s, la, lb,
-                           ep, M, C, T * cin, T * cin);
+        EpConvFwd<PP> ep{e->theta_b + boff[l], e->blen, train ? e->H[l] : nullptr, out};
+        hipLaunchKernelGGL((k_gemm<LdConv<PP>, LdRows, EpConvFwd<PP>>), dim3((M + GM - 1) / GM, 1, 1), dim3(256), 0, s,
+                           la, lb, ep, M, C, T * cin, T * cin);
         in = out;
         cin = C;
     }
-    hipLaunchKernelGGL(k_head_fwd, dim3(nblocks(nseq, 4)), dim3(256), 0, s, in, e->blen, nseq, e->theta_b + O_WP,
+    hipLaunchKernelGGL(k_head_fwd, dim3(nblocks(nseq, 4)), dim3(256), 0, s, in, e->blen, nseq, PP, e->theta_b + O_WP,
                        e->theta_b + O_BP, train ? e->pool : nullptr, train ? e->u : nullptr, e->z, zb_out);
     hcheck(hipGetLastError(), "encoder forward launch");
 }
 
+void forward(idb_encoder *e, uint32_t nseq, bool train, uint16_t *zb_out)
+{
+    if (e->P == 160) forward_p<160>(e, nseq, train, zb_out);
+    else forward_p<32>(e, nseq, train, zb_out);
+}
+
 // Backward of sum(dz . z) into e->grad (zeroed here).
-void backward(idb_encoder *e, uint32_t nseq)
+template <uint32_t PP>
+void backward_p(idb_encoder *e, uint32_t nseq)
 {
     hipStream_t s = e->stream;
-    const uint32_t M = nseq * P;
+    const uint32_t M = nseq * PP;
     hcheck(hipMemsetAsync(e->grad, 0, sizeof(float) * N_PARAMS, s), "memset");
-    hipLaunchKernelGGL(k_head_bwd, dim3(nblocks(nseq, 4)), dim3(256), 0, s, e->dz, e->z, e->u, e->blen, nseq,
+    hipLaunchKernelGGL(k_head_bwd, dim3(nblocks(nseq, 4)), dim3(256), 0, s, e->dz, e->z, e->u, e->blen, nseq, PP,
                        e->theta_b + O_WP, e->H[2], e->du, e->dHa);
     hipLaunchKernelGGL(k_proj_grad, dim3(D, 64), dim3(C), 0, s, e->du, e->pool, nseq, e->grad + O_WP, e->grad + O_BP);
     const uint32_t offs[3] = {O_W1, O_W2, O_W3}, boff[3] = {O_B1, O_B2, O_B3}, cins[3] = {C0, C, C};
@@ -684,20 +704,26 @@ void backward(idb_encoder *e, uint32_t nseq)
         const uint32_t cin = cins[l];
         const uint16_t *Aprev = l ? e->A[l - 1] : e->X0;
         LdGradT la{dH, M};
-        LdConvT lb{Aprev, cin, M};
+        LdConvT<PP> lb{Aprev, cin, M};
         EpAtomic ep{e->grad + offs[l], T * cin};
-        hipLaunchKernelGGL((k_gemm<LdGradT, LdConvT, EpAtomic>), dim3(1, (T * cin + GN - 1) / GN, nsplit), dim3(256), 0,
-                           s, la, lb, ep, C, T * cin, M, ks);
+        hipLaunchKernelGGL((k_gemm<LdGradT, LdConvT<PP>, EpAtomic>), dim3(1, (T * cin + GN - 1) / GN, nsplit), dim3(256),
+                           0, s, la, lb, ep, C, T * cin, M, ks);
         hipLaunchKernelGGL(k_bias_grad, dim3(2048), dim3(C), 0, s, dH, M, e->grad + boff[l]);
-        LdConv la2{dH, C, M, 1};
+        LdConv<PP> la2{dH, C, M, 1};
         LdRows lb2{e->wt[l], T * C, cin};
-        EpConvBwd ep2{l ? e->H[l - 1] : nullptr, e->blen, cin, l ? dHn : nullptr, l ? nullptr : e->dX0};
-        hipLaunchKernelGGL((k_gemm<LdConv, LdRows, EpConvBwd>), dim3((M + GM - 1) / GM, (cin + GN - 1) / GN, 1),
+        EpConvBwd<PP> ep2{l ? e->H[l - 1] : nullptr, e->blen, cin, l ? dHn : nullptr, l ? nullptr : e->dX0};
+        hipLaunchKernelGGL((k_gemm<LdConv<PP>, LdRows, EpConvBwd<PP>>), dim3((M + GM - 1) / GM, (cin + GN - 1) / GN, 1),
                            dim3(256), 0, s, la2, lb2, ep2, M, cin, T * C, T * C);
         std::swap(dH, dHn);
     }
-    hipLaunchKernelGGL(k_emb_grad, dim3(256), dim3(256), 0, s, e->dX0, e->btok, e->blen, M, e->grad + O_EMB);
+    hipLaunchKernelGGL(k_emb_grad, dim3(256), dim3(256), 0, s, e->dX0, e->btok, e->blen, M, PP, e->grad + O_EMB);
     hcheck(hipGetLastError(), "encoder backward launch");
+}
+
+void backward(idb_encoder *e, uint32_t nseq)
+{
+    if (e->P == 160) backward_p<160>(e, nseq);
+    else backward_p<32>(e, nseq);
 }
 
 void tokenize(idb_encoder *e, const idb_dataset *d)
@@ -711,12 +737,12 @@ void tokenize(idb_encoder *e, const idb_dataset *d)
         e->tok = nullptr;
         e->tlen = nullptr;
         e->tok_cap = 0;
-        hcheck(hipMalloc(&e->tok, (size_t)n * P), "hipMalloc");
+        hcheck(hipMalloc(&e->tok, (size_t)n * e->P), "hipMalloc");
         hcheck(hipMalloc(&e->tlen, sizeof(uint32_t) * n), "hipMalloc");
         e->tok_cap = n;
     }
     hipLaunchKernelGGL(k_tokenize, dim3(nblocks(n)), dim3(256), 0, e->stream, d->seq, d->off, d->len, n, e->vocab_d,
-                       e->tok, e->tlen);
+                       e->P, e->tok, e->tlen);
     hcheck(hipGetLastError(), "tokenize launch");
 }
 
@@ -761,8 +787,8 @@ float step(idb_encoder *e, const uint32_t *pairs_d, const float *labels_d, const
     hipLaunchKernelGGL(k_iota_rows, dim3(nblocks(nseq)), dim3(256), 0, s, 0u, 0u, nseq, e->brows);
     hipLaunchKernelGGL(k_pair_rows, dim3(nblocks(nb)), dim3(256), 0, s, pairs_d, order_d, first, nb, npairs, e->brows,
                        e->y, labels_d);
-    hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(nseq)), dim3(256), 0, s, e->tok, e->tlen, e->brows, nseq, e->btok,
-                       e->blen);
+    hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(nseq)), dim3(256), 0, s, e->tok, e->tlen, e->brows, nseq, e->P,
+                       e->btok, e->blen);
     forward(e, nseq, true, nullptr);
     hcheck(hipMemsetAsync(e->loss, 0, sizeof(float), s), "memset");
     hcheck(hipMemsetAsync(e->dz, 0, sizeof(float) * (size_t)nseq * D, s), "memset");
@@ -810,13 +836,14 @@ hipError_t idb_encoder_embed_rows(idb_encoder *e, const uint8_t *seq, const uint
         e->stream = s;
         try {
             tokenize(e, &d);
-            const uint32_t chunk = 32768;
+            // 32768 rows of 32 positions per chunk: activations of 1M positions
+            const uint32_t chunk = round4(32768u * 32u / e->P);
             ensure_workspace(e, std::min(round4(n), chunk), false);
             for (uint32_t first = 0; first < n; first += chunk) {
                 const uint32_t nb = round4(std::min(chunk, n - first));
                 hipLaunchKernelGGL(k_iota_rows, dim3(nblocks(nb)), dim3(256), 0, s, first, n, nb, e->brows);
                 hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(nb)), dim3(256), 0, s, e->tok, e->tlen, e->brows, nb,
-                                   e->btok, e->blen);
+                                   e->P, e->btok, e->blen);
                 forward(e, nb, false, zb + (size_t)first * D);
             }
         } catch (...) {
@@ -829,6 +856,7 @@ hipError_t idb_encoder_embed_rows(idb_encoder *e, const uint8_t *seq, const uint
 }
 
 const char *idb_encoder_error(const idb_encoder *e) { return e ? e->err.c_str() : "no encoder"; }
+uint32_t idb_encoder_positions(const idb_encoder *e) { return e ? e->P : 0u; }
 int idb_encoder_device(const idb_encoder *e) { return e ? e->device : -1; }
 
 extern "C" {
@@ -837,11 +865,19 @@ uint32_t idb_encoder_param_count(void) { return N_PARAMS; }
 
 int idb_encoder_create(int device, const uint8_t *vocab, uint64_t seed, idb_encoder **out)
 {
+    return idb_encoder_create_positions(device, vocab, IDB_ENC_P, seed, out);
+}
+
+int idb_encoder_create_positions(int device, const uint8_t *vocab, uint32_t positions, uint64_t seed,
+                                 idb_encoder **out)
+{
     if (!out || !vocab) return IDB_ERR_ARG;
     *out = nullptr;
+    if (positions != 32 && positions != 160) return IDB_ERR_UNSUPPORTED;
     auto *e = new (std::nothrow) idb_encoder();
     if (!e) return IDB_ERR_OOM;
     e->device = device;
+    e->P = positions;
     std::memcpy(e->vocab, vocab, 256);
     for (int b = 0; b < 256; ++b)
         if (e->vocab[b] >= NTOK) {
@@ -934,14 +970,14 @@ int idb_encoder_embed(idb_encoder *e, const idb_dataset *d, float *z_host)
         DevData dd;
         const idb_dataset *dv = device_view(d, dd, e->stream);
         tokenize(e, dv);
-        const uint32_t n = d->n_rows, chunk = 32768;
+        const uint32_t n = d->n_rows, chunk = round4(32768u * 32u / e->P);
         ensure_workspace(e, std::min(round4(std::max(n, 1u)), chunk), false);
         std::vector<float> buf;
         for (uint32_t first = 0; first < n; first += chunk) {
             const uint32_t cnt = std::min(chunk, n - first), nb = round4(cnt);
             hipLaunchKernelGGL(k_iota_rows, dim3(nblocks(nb)), dim3(256), 0, e->stream, first, n, nb, e->brows);
             hipLaunchKernelGGL(k_gather_rows, dim3(nblocks(nb)), dim3(256), 0, e->stream, e->tok, e->tlen, e->brows, nb,
-                               e->btok, e->blen);
+                               e->P, e->btok, e->blen);
             forward(e, nb, false, nullptr);
             hcheck(hipMemcpyAsync(z_host + (size_t)first * D, e->z, sizeof(float) * cnt * D, hipMemcpyDeviceToHost,
                                   e->stream), "copy");

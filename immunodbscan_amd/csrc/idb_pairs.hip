@@ -1491,13 +1491,13 @@ __device__ __forceinline__ idb_bf16x8 load_z8(const uint16_t *zb, uint32_t row, 
     return __builtin_bit_cast(idb_bf16x8, v);
 }
 
+template <int W>
 __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, kept in a scalar register
     const uint32_t K = a.K;
-    constexpr int W = 1;
     // per wave: Peq, edge stage, queue, count accumulators, query records
     uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
     EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
@@ -1575,7 +1575,9 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
         bool peq_ready = false;
         uint32_t qlen = 0;
         auto build_peq = [&]() {
-            for (uint32_t c = 0; c < K; ++c) peq[c * 64 + lane] = 0u;
+            for (uint32_t c = 0; c < K; ++c)
+#pragma unroll
+                for (int w = 0; w < W; ++w) peq[(c * W + w) * 64 + lane] = 0u;
             if (tvalid) {
                 const uint4 *codes = (const uint4 *)(a.e_codes + (size_t)p * a.S);
                 const uint32_t m = te.len_chk & 0xFFFFu;
@@ -1585,7 +1587,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
 #pragma unroll
                     for (uint32_t k = 0; k < 16; ++k) {
                         const uint32_t i = i0 + k;
-                        if (i < m) peq[((ws[k >> 2] >> (8 * (k & 3))) & 0xFFu) * 64 + lane] |= 1u << (i & 31);
+                        if (i < m) peq[(((ws[k >> 2] >> (8 * (k & 3))) & 0xFFu) * W + (i >> 5)) * 64 + lane] |= 1u << (i & 31);
                     }
                 }
             }
@@ -1629,8 +1631,8 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
             if (qlen >= IDB_TILE) {
                 if (!peq_ready) build_peq();
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                myers_batch<W, false>(a, tile, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits,
-                                      mletters);
+                myers_batch_any<W, false>(a, tile, peq, queue, IDB_TILE, lane, qe, te, 0, 0, est, qacc, tacc, hits,
+                                          mletters);
                 const uint32_t rest = qlen - IDB_TILE;
                 const uint16_t moved = lane < rest ? queue[IDB_TILE + lane] : (uint16_t)0;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1641,7 +1643,7 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
         if (qlen) {
             if (!peq_ready) build_peq();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            myers_batch<W, false>(a, tile, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters);
+            myers_batch_any<W, false>(a, tile, peq, queue, qlen, lane, qe, te, 0, 0, est, qacc, tacc, hits, mletters);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const int32_t qa = qacc[lane], ta = tacc[lane];
@@ -2195,19 +2197,32 @@ hipError_t launch_learned(const idb_pair_args &a_in, hipStream_t s)
         a.tune_grid_mul = 4;
         a.no_xcd_order = 1;
     }
-    const size_t per_wave = idb_myers_lds_per_wave(a.K, 1);
+    // pattern words of the longest row (the encoder reads up to 160 letters)
+    const uint32_t W = std::max(1u, (a.S + 31) / 32);
+    if (W > 5) return hipErrorInvalidValue;
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, W);
     uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
     if (wpb > 4) wpb = 4;
     if (wpb == 0) return hipErrorInvalidValue;
     wpb = pick_wpb(a, wpb);
     const size_t shm = per_wave * wpb;
-    const void *k = (const void *)k_pairs_learned;
+    const void *k = W == 1   ? (const void *)k_pairs_learned<1>
+                    : W == 2 ? (const void *)k_pairs_learned<2>
+                    : W == 3 ? (const void *)k_pairs_learned<3>
+                    : W == 4 ? (const void *)k_pairs_learned<4>
+                             : (const void *)k_pairs_learned<5>;
     if (shm > 65536) {
         const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
     const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, block.x, shm), a.tune_grid_mul));
-    hipLaunchKernelGGL(k_pairs_learned, grid, block, shm, s, a, wpb);
+    switch (W) {
+    case 1: hipLaunchKernelGGL(k_pairs_learned<1>, grid, block, shm, s, a, wpb); break;
+    case 2: hipLaunchKernelGGL(k_pairs_learned<2>, grid, block, shm, s, a, wpb); break;
+    case 3: hipLaunchKernelGGL(k_pairs_learned<3>, grid, block, shm, s, a, wpb); break;
+    case 4: hipLaunchKernelGGL(k_pairs_learned<4>, grid, block, shm, s, a, wpb); break;
+    default: hipLaunchKernelGGL(k_pairs_learned<5>, grid, block, shm, s, a, wpb); break;
+    }
     return hipGetLastError();
 }
 

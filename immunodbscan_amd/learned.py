@@ -44,14 +44,17 @@ class TrainParams:
 class Encoder:
     """An idb_encoder: parameters and optimiser state on one device."""
 
-    def __init__(self, vocab: np.ndarray, seed: int = 0, device: int = 0):
+    def __init__(self, vocab: np.ndarray, seed: int = 0, device: int = 0, positions: int = P):
+        """positions: letters per row the encoder reads, 32 (protein rows) or
+        160 (nucleotide rows of up to 160 letters)."""
         self.vocab = np.ascontiguousarray(vocab, np.uint8)
         h = C.c_void_p()
-        rc = _lib.lib.idb_encoder_create(device, self.vocab.ctypes.data, seed, C.byref(h))
+        rc = _lib.lib.idb_encoder_create_positions(device, self.vocab.ctypes.data, positions, seed, C.byref(h))
         if rc != _lib.IDB_OK:
-            raise _lib.IdbError(rc, "idb_encoder_create failed")
+            raise _lib.IdbError(rc, "idb_encoder_create_positions failed")
         self._h = h
         self.device = device
+        self.positions = positions
 
     def close(self) -> None:
         if getattr(self, "_h", None):

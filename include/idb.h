@@ -204,7 +204,8 @@ int idb_neighbour_pairs(idb_ctx *ctx, const idb_dataset *data, const idb_params 
  * Learned candidate filter.  No reference counterpart.  A small 1-D CNN
  * (token embedding 32 -> 3 x conv1d(k=5, 128 channels, GELU) -> mean over the
  * row's letters -> linear 128 -> 64 -> L2 norm; bfloat16 weights, float32
- * accumulation) maps each row (first 32 letters) to a unit vector.  The
+ * accumulation) maps each row (its first P letters, P = 32 or 160) to a unit
+ * vector.  The
  * learned clustering run proposes, inside the same V/J bucket tiles as the
  * exact run, the pairs whose embedding dot product reaches min_similarity,
  * and verifies each proposal with the exact Myers kernel: distances are never
@@ -233,6 +234,12 @@ typedef struct {
 /* vocab maps every byte value to a token id below 32.  Parameters are
  * initialised from seed (N(0,1) embedding, U(+-1/sqrt(fan_in)) weights). */
 int idb_encoder_create(int device, const uint8_t *vocab, uint64_t seed, idb_encoder **enc);
+/* The same with P positions per row: 32 (protein rows) or 160 (nucleotide
+ * rows of up to 160 letters); the parameters do not depend on P.  A learned
+ * run needs rows of at most P letters. */
+int idb_encoder_create_positions(int device, const uint8_t *vocab, uint32_t positions, uint64_t seed,
+                                 idb_encoder **enc);
+uint32_t idb_encoder_positions(const idb_encoder *enc);
 void idb_encoder_destroy(idb_encoder *enc);
 const char *idb_encoder_last_error(const idb_encoder *enc);
 uint32_t idb_encoder_param_count(void);

@@ -8,7 +8,8 @@ CDR3 letters that the learned filter uses to propose neighbour pairs:
            -> 3 x [conv1d(kernel 5, 'same' zero padding, 128 channels) + GELU]
            -> mean over the row's letters -> linear 128 -> 64 -> L2 normalise.
 
-A row occupies P = 32 positions; positions at or past its length hold zeros
+A row occupies P positions (32 by default; the token matrix's width sets
+it); positions at or past its length hold zeros
 after every layer, so each row is convolved with its own zero padding.  GELU
 is the tanh form 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
 
@@ -85,10 +86,10 @@ def gelu_grad(x):
 
 def _im2col(X: np.ndarray) -> np.ndarray:
     """[n, P, Cin] -> [n, P, T*Cin] with column t*Cin + c = X[:, p + t - 2, c]."""
-    n, _, cin = X.shape
-    Xp = np.zeros((n, P + T - 1, cin), X.dtype)
-    Xp[:, 2:2 + P] = X
-    return np.concatenate([Xp[:, t:t + P] for t in range(T)], axis=2)
+    n, npos, cin = X.shape
+    Xp = np.zeros((n, npos + T - 1, cin), X.dtype)
+    Xp[:, 2:2 + npos] = X
+    return np.concatenate([Xp[:, t:t + npos] for t in range(T)], axis=2)
 
 
 def conv(X: np.ndarray, Wt: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -97,12 +98,12 @@ def conv(X: np.ndarray, Wt: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 def conv_backward_data(dH: np.ndarray, Wt: np.ndarray, cin: int) -> np.ndarray:
     """dX[q, c] = sum_{t, o} dH[q - t + 2, o] Wt[o, t*cin + c]."""
-    n = dH.shape[0]
+    n, npos = dH.shape[0], dH.shape[1]
     dcol = dH @ Wt  # [n, P, T*cin]: gradient w.r.t. im2col columns
-    dX = np.zeros((n, P + T - 1, cin), dH.dtype)
+    dX = np.zeros((n, npos + T - 1, cin), dH.dtype)
     for t in range(T):
-        dX[:, t:t + P] += dcol[:, :, t * cin:(t + 1) * cin]
-    return dX[:, 2:2 + P]
+        dX[:, t:t + npos] += dcol[:, :, t * cin:(t + 1) * cin]
+    return dX[:, 2:2 + npos]
 
 
 def forward(flat: np.ndarray, tok: np.ndarray, lens: np.ndarray, emulate_bf16: bool = True,
@@ -113,7 +114,7 @@ def forward(flat: np.ndarray, tok: np.ndarray, lens: np.ndarray, emulate_bf16: b
     p = {k: (rnd(v.astype(np.float32)) if emulate_bf16 else v).astype(dtype) for k, v in unflatten(flat).items()}
     n = tok.shape[0]
     lens = np.asarray(lens, np.int64)
-    mask = (np.arange(P)[None, :] < lens[:, None]).astype(dtype)[:, :, None]
+    mask = (np.arange(tok.shape[1])[None, :] < lens[:, None]).astype(dtype)[:, :, None]
     X0 = p["emb"][tok] * mask
     cache = {"p": p, "mask": mask, "lens": lens, "tok": tok, "X0": X0, "rnd": rnd}
     A = X0
@@ -180,12 +181,13 @@ def loss_and_grad(flat, tok, lens, ia, ib, y, margin, emulate_bf16=True, dtype=n
     return loss, backward(cache, dz.astype(dtype)), cache
 
 
-def tokens_of(seqs: list[bytes], vocab: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    """Token matrix [n, P] (0 past the end) and lengths (at most P)."""
-    tok = np.zeros((len(seqs), P), np.int64)
+def tokens_of(seqs: list[bytes], vocab: np.ndarray, positions: int = P) -> tuple[np.ndarray, np.ndarray]:
+    """Token matrix [n, positions] (0 past the end) and lengths (at most
+    positions)."""
+    tok = np.zeros((len(seqs), positions), np.int64)
     lens = np.zeros(len(seqs), np.int64)
     for i, s in enumerate(seqs):
-        b = np.frombuffer(s[:P], np.uint8)
+        b = np.frombuffer(s[:positions], np.uint8)
         tok[i, :len(b)] = vocab[b]
         lens[i] = len(b)
     return tok, lens

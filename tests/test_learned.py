@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from immunodbscan_amd import configs, synth
+from immunodbscan_amd import _lib, configs, synth
 from oracle import learned as Lr
 
 
@@ -147,6 +147,62 @@ def test_encoder_gradients_match_oracle(gpu_ctx):
         cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
         print(name, "rel", _rel(a, b), "cos", cos, flush=True)
         assert _rel(a, b) < 2e-2 and cos > 0.999, name
+
+
+def _nt_rep(n, seeds, seed):
+    """Nucleotide rows of 24-150 letters, as the 20M-row workload draws them."""
+    return synth.generate(configs.c5_params(n_rows=n, n_seeds=seeds, seed=seed))
+
+
+@pytest.mark.gpu
+def test_encoder_160_positions_forward_and_gradients_match_oracle(gpu_ctx):
+    """The nucleotide encoder: 160 positions per row (rows of 24-150 letters)."""
+    from immunodbscan_amd import learned
+
+    rep = _nt_rep(64, 64, 51)
+    enc = learned.Encoder(learned.vocab_for(synth.LETTERS[synth.NT]), seed=5, positions=160)
+    assert _lib.lib.idb_encoder_positions(enc._h) == 160
+    flat = enc.get_params()
+    z = enc.embed(rep)
+    seqs = [bytes(rep.seq[rep.off[i]:rep.off[i] + rep.length[i]]) for i in range(rep.n)]
+    tok, lens = Lr.tokens_of(seqs, enc.vocab, positions=160)
+    assert tok.shape == (64, 160) and lens.max() > 96
+    ref = Lr.forward(flat, tok, lens, emulate_bf16=True)["z"]
+    print("forward max abs diff", np.abs(z - ref).max(), flush=True)
+    assert np.abs(z - ref).max() < 1e-2 and np.abs(z - ref).mean() < 1e-3
+    rng = np.random.default_rng(8)
+    pairs = rng.integers(0, rep.n, (24, 2)).astype(np.uint32)
+    labels = (rng.random(24) < 0.5).astype(np.float32)
+    loss, g = enc.loss_grad(rep, pairs, labels, 0.9)
+    lref, gref, _ = Lr.loss_and_grad(flat, tok, lens, pairs[:, 0].astype(np.int64), pairs[:, 1].astype(np.int64),
+                                     labels, 0.9, emulate_bf16=True)
+    assert abs(loss - lref) <= 1e-3 * abs(lref) + 1e-6
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in Lr.SHAPES])
+    for k, (name, _) in enumerate(Lr.SHAPES):
+        a, b = g[offs[k]:offs[k + 1]], gref[offs[k]:offs[k + 1]]
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        assert _rel(a, b) < 2e-2 and cos > 0.999, name
+    # rows longer than the encoder reads are refused by a learned run
+    short = learned.Encoder(learned.vocab_for(synth.LETTERS[synth.NT]), seed=5)
+    with pytest.raises(_lib.IdbError) as e:
+        learned.dbscan_learned(rep, configs.C5, short, -2.0)
+    assert e.value.code == _lib.IDB_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+def test_learned_run_on_long_rows_proposing_everything_equals_exact(gpu_ctx):
+    """Rows of up to 5 pattern words: every gated pair proposed, verified by
+    the multi-word (banded) Myers batches, gives the exact run's results."""
+    from immunodbscan_amd import engine, learned
+
+    rep = _nt_rep(6_000, 300, 52)
+    enc = learned.Encoder(learned.vocab_for(synth.LETTERS[synth.NT]), seed=1, positions=160)
+    ex = engine.dbscan(rep, configs.C5)
+    le = learned.dbscan_learned(rep, configs.C5, enc, -2.0)
+    for f in ("clone_id", "core", "copies", "counts"):
+        assert np.array_equal(getattr(ex, f), getattr(le, f)), f
+    assert le.stats["neighbour_pairs"] == ex.stats["neighbour_pairs"] > 0
+    assert le.stats["gated_pairs"] == ex.stats["gated_pairs"]
 
 
 @pytest.mark.gpu
