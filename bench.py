@@ -87,6 +87,7 @@ def filter_ops(s: dict) -> int:
 
 
 ENC_FLOPS_PER_ROW = 2 * 32 * (5 * 32 * 128 + 2 * 5 * 128 * 128)  # three conv layers, 32 positions
+ENC160_FLOPS_PER_ROW = 5 * ENC_FLOPS_PER_ROW  # the nucleotide encoder: 160 positions
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -351,6 +352,60 @@ def large_section(args, threads: int) -> dict:
                          "sample": f"literal all-pairs clustering of the first {sub_n} rows in {dt:.1f} s; "
                                    "extrapolated"},
         "gpu_over_cpu": (n * (n - 1) / 2 / (ms * 1e-3)) / cpu_v,
+        **({"learned": large_learned(args, rep, ms, run.result())} if args.c5_learned else {}),
+    }
+
+
+def large_learned(args, rep, exact_ms, exact_res) -> dict:
+    """The learned filter on the same 20M nucleotide rows, side by side with
+    the exact path: a 160-position encoder trained on pairs of a separate
+    repertoire of the same generator, its threshold set on a third."""
+    from immunodbscan_amd import configs, engine, learned, synth
+
+    cl = configs.C5
+    t0 = time.perf_counter()
+    vocab = learned.vocab_for(synth.LETTERS[synth.NT])
+    train = synth.generate(configs.c5_params(n_rows=200_000, n_seeds=1_000, seed=61))
+    pairs, labels = learned.training_pairs(train, cl, args.c5_train_pairs, seed=3)
+    enc = learned.Encoder(vocab, seed=2, positions=160)
+    tp = learned.TrainParams(epochs=args.train_epochs, batch_pairs=4096, lr=5e-4, margin=0.9)
+    tstats = enc.train(train, pairs, labels, tp)
+    tune = synth.generate(configs.c5_params(n_rows=200_000, n_seeds=1_000, seed=63))
+    tau = learned.tune_threshold(enc.embed(tune), learned.neighbour_pairs(tune, cl), recall=0.9998, slack=0.01)
+    log(f"large learned: trained {tstats['steps']} steps in {tstats['train_ms']:.0f} ms, tau {tau:.4f} "
+        f"({time.perf_counter() - t0:.1f} s with labelling)")
+    lrun = engine.DeviceRun(rep, cl, encoder=enc, min_sim=tau)
+    ms, _ = timed(lrun, 1, 1)
+    res = lrun.result()
+    s = res.stats
+    enc_ms = s["encoder_ms"]
+    flops = ENC160_FLOPS_PER_ROW * rep.n
+    log(f"large learned: {ms:.1f} ms/step against exact {exact_ms:.1f}")
+    return {
+        "training": {"pairs": int(len(labels)), "positive_fraction": float(labels.mean()), **tstats,
+                     "repertoire": "200k rows of the same generator (seed 61)"},
+        "min_similarity": tau,
+        "threshold_rule": "0.02% quantile of neighbour-pair similarities on a separate 200k-row repertoire, "
+                          "less 0.01 for bfloat16 rounding",
+        "filter_after_proposals": "the exact 3-gram profile bound, then Myers",
+        "ms_per_step": ms,
+        "exact_ms_per_step": exact_ms,
+        "speedup_vs_exact": exact_ms / ms,
+        "pair_recall": s["neighbour_pairs"] / max(exact_res.stats["neighbour_pairs"], 1),
+        "ari": learned.ari(exact_res.clone_id, res.clone_id),
+        "proposals": s["gated_pairs"],
+        "exact_gated_pairs": exact_res.stats["gated_pairs"],
+        "myers_pairs": s["myers_pairs"],
+        "exact_myers_pairs": exact_res.stats["myers_pairs"],
+        "stage_ms": s["stage_ms"],
+        "exact_stage_ms": exact_res.stats["stage_ms"],
+        "count_kernel_ms": s["count_kernel_ms"],
+        "exact_count_kernel_ms": exact_res.stats["count_kernel_ms"],
+        "encoder_ms": enc_ms,
+        "encoder_roofline": {"bound": "mfma", "achieved": flops / (enc_ms * 1e-3) / 1e12,
+                             "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": flops / (enc_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+                             "flops_per_row": ENC160_FLOPS_PER_ROW},
     }
 
 
@@ -439,6 +494,9 @@ def main() -> int:
     ap.add_argument("--c3-reps", type=int, default=512)
     ap.add_argument("--c5-rows", type=int, default=20_000_000, help="0 skips the 20M-row nucleotide line")
     ap.add_argument("--c5-cpu-rows", type=int, default=40_000)
+    ap.add_argument("--no-c5-learned", dest="c5_learned", action="store_false",
+                    help="skip the learned filter on the 20M-row nucleotide line")
+    ap.add_argument("--c5-train-pairs", type=int, default=1_000_000)
     ap.add_argument("--gene-rows", type=int, default=10_000_000, help="0 skips the device gene-bitset line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the timed steps, run one device-to-device copy of this many MiB "

@@ -1317,8 +1317,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // The absent-letter filter pays off where random letter matches are
         // rare (protein alphabets); with 4 letters nearly every pair passes.
         const bool filt = !ham && W == 1 && K >= 8 && K <= 31;
-        // Nucleotide rows: the 3-gram profile bound (learned runs use their own proposals).
-        const bool qgram = !ham && !learned && K <= 4 && maxlen >= 16;
+        // Nucleotide rows: the 3-gram profile bound (learned runs apply it to their proposals).
+        const bool qgram = !ham && K <= 4 && maxlen >= 16;
         uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
         uint32_t *par_e = buf<uint32_t>(c, S_PARE, E), *cmin_e = buf<uint32_t>(c, S_CMINE, E);
         // The Levenshtein count passes test the canonical bucket on the

@@ -1498,8 +1498,12 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
     const uint32_t lane = lane_id();
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, kept in a scalar register
     const uint32_t K = a.K;
-    // per wave: Peq, edge stage, queue, count accumulators, query records
-    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t));
+    // per wave: Peq, edge stage, queue, count accumulators, query records,
+    // then (3-gram bound) the tile's query and target profiles
+    const uint32_t QGW = a.qgram_filter ? 2u * IDB_TILE * IDB_QGRAM_WORDS : 0u;
+    uint32_t *peq = lds + (size_t)wib * (idb_myers_lds_per_wave(K, W) / sizeof(uint32_t) + QGW);
+    uint32_t *qgp = peq + idb_myers_lds_per_wave(K, W) / sizeof(uint32_t);
+    uint32_t *tgp = qgp + IDB_TILE * IDB_QGRAM_WORDS;
     EdgeStage est{(uint2 *)(peq + K * W * 64), 0u, IDB_NONE, 0u, 0u};
     uint16_t *queue = (uint16_t *)(peq + K * W * 64 + 2 * IDB_TILE);
     int32_t *qacc = (int32_t *)(peq + K * W * 64 + 3 * IDB_TILE);
@@ -1521,6 +1525,16 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
         qrec[lane] = qe;
         qacc[lane] = 0;
         tacc[lane] = 0;
+        if (a.qgram_filter) {
+            // profiles of the tile's queries and targets (64 bytes each)
+            const uint4 *qs = (const uint4 *)(a.e_qg + (size_t)(tile.q0 + (lane < qn ? lane : 0u)) * IDB_QGRAM_WORDS);
+            const uint4 *ts = (const uint4 *)(a.e_qg + (size_t)p * IDB_QGRAM_WORDS);
+#pragma unroll
+            for (int k = 0; k < (int)IDB_QGRAM_WORDS / 4; ++k) {
+                ((uint4 *)(qgp + lane * IDB_QGRAM_WORDS))[k] = qs[k];
+                ((uint4 *)(tgp + lane * IDB_QGRAM_WORDS))[k] = ts[k];
+            }
+        }
         // rows of the 32-row halves this lane feeds into the MFMAs
         const uint32_t h = lane & 31u;
         const uint32_t rq0 = (uint32_t)__shfl((int)qe.row, (int)h, 64), rq1 = (uint32_t)__shfl((int)qe.row, (int)(32 + h), 64);
@@ -1617,6 +1631,20 @@ __global__ __launch_bounds__(256) void k_pairs_learned(idb_pair_args a, uint32_t
                     const uint32_t lbx = __builtin_popcount(q.pres1 & ~p1) + __builtin_popcount(q.pres2 & ~p1);
                     const uint32_t lby = __builtin_popcount(p1 & ~q.pres1) + __builtin_popcount(p2 & ~q.pres1);
                     ok = (int32_t)(lbx > lby ? lbx : lby) <= t;
+                } else if (ok && a.qgram_filter) {
+                    // the exact 3-gram bound, as the exact pass applies it
+                    const uint4 *qv = (const uint4 *)(qgp + qi * IDB_QGRAM_WORDS);
+                    const uint4 *tv = (const uint4 *)(tgp + tl * IDB_QGRAM_WORDS);
+                    uint32_t l1 = 0;
+#pragma unroll
+                    for (int k = 0; k < (int)IDB_QGRAM_WORDS / 4; ++k) {
+                        const uint4 x4 = qv[k], y4 = tv[k];
+                        l1 = __builtin_amdgcn_sad_u8(x4.x, y4.x, l1);
+                        l1 = __builtin_amdgcn_sad_u8(x4.y, y4.y, l1);
+                        l1 = __builtin_amdgcn_sad_u8(x4.z, y4.z, l1);
+                        l1 = __builtin_amdgcn_sad_u8(x4.w, y4.w, l1);
+                    }
+                    ok = (int32_t)l1 <= 6 * t;
                 }
             }
             gated += __popcll(__ballot(proposed));
@@ -2200,7 +2228,7 @@ hipError_t launch_learned(const idb_pair_args &a_in, hipStream_t s)
     // pattern words of the longest row (the encoder reads up to 160 letters)
     const uint32_t W = std::max(1u, (a.S + 31) / 32);
     if (W > 5) return hipErrorInvalidValue;
-    const size_t per_wave = idb_myers_lds_per_wave(a.K, W);
+    const size_t per_wave = idb_myers_lds_per_wave(a.K, W) + (a.qgram_filter ? 2 * idb_qgram_lds_per_wave() : 0);
     uint32_t wpb = (uint32_t)(std::min<size_t>(idb_lds_limit(), 65536) / per_wave);
     if (wpb > 4) wpb = 4;
     if (wpb == 0) return hipErrorInvalidValue;

@@ -119,13 +119,15 @@ def pair_distances(rep: Repertoire, pairs: np.ndarray, hamming: bool = False, ct
 
 
 def neighbour_pairs(rep: Repertoire, cl: Clustering, ctx=None, cap: int | None = None) -> np.ndarray:
-    """Unordered unique-row pairs (x < y) with N(x, y), from the exact count pass."""
+    """Unordered unique-row pairs (x < y) with N(x, y), from the exact count
+    pass; the pass's own pair list is given the same capacity (dense clones,
+    as in the nucleotide workloads, hold tens of pairs per row)."""
     ctx = ctx or default_context()
-    cap = cap if cap is not None else max(1 << 20, 16 * rep.n)
+    cap = cap if cap is not None else min(max(1 << 22, 256 * rep.n), 1 << 28)
     out = np.zeros((cap, 2), np.uint32)
     n = C.c_uint64()
-    ctx.check(_lib.lib.idb_neighbour_pairs(ctx.handle, _dataset(rep), _params(cl, False, 0), out.ctypes.data, cap,
-                                           C.byref(n)))
+    ctx.check(_lib.lib.idb_neighbour_pairs(ctx.handle, _dataset(rep), _params(cl, False, 0, edge_capacity=cap),
+                                           out.ctypes.data, cap, C.byref(n)))
     return out[: n.value].copy()
 
 

@@ -203,6 +203,8 @@ def test_learned_run_on_long_rows_proposing_everything_equals_exact(gpu_ctx):
         assert np.array_equal(getattr(ex, f), getattr(le, f)), f
     assert le.stats["neighbour_pairs"] == ex.stats["neighbour_pairs"] > 0
     assert le.stats["gated_pairs"] == ex.stats["gated_pairs"]
+    # the same exact 3-gram bound before Myers
+    assert le.stats["myers_pairs"] == ex.stats["myers_pairs"]
 
 
 @pytest.mark.gpu
