@@ -25,6 +25,7 @@ IDB_N_STAGES = 8
 IDB_FLAG_AUTOTUNE = 1
 IDB_FLAG_GRAPH = 2
 IDB_FLAG_FORCE_LARGE_PATHS = 4
+IDB_FLAG_SPLIT_TILES = 8
 IDB_NOALLOC_MAX_ROWS = 65536
 
 STAGE_NAMES = [

@@ -29,6 +29,9 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <atomic>
+#include <condition_variable>
+#include <thread>
 #include <vector>
 
 #include <rocprofiler-sdk-roctx/roctx.h>
@@ -83,6 +86,9 @@ struct idb_ctx {
     void *ptr[S_COUNT] = {};
     size_t cap[S_COUNT] = {};
     uint64_t *pinned = nullptr;   // pinned host staging: read-backs [0, 32 KiB), uploads [32, 64 KiB)
+    // Pinned staging of host-memory inputs and outputs (grown on demand).
+    uint8_t *xfer = nullptr;
+    size_t xfer_cap = 0;
     hipEvent_t ev[IDB_N_STAGES + 1] = {};
     hipEvent_t kev[4] = {};       // around the count and the fuse kernels
     // autotuner: best count-kernel launch shape per kernel and shape class
@@ -696,7 +702,8 @@ __device__ __forceinline__ int64_t last_target_block(const QBlock &q, const uint
 // entries (single-bucket mode at 20M rows) makes more than 2^32 tiles, and
 // their total is scanned in 64 bits so that the host sees it whole.
 __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
-                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint64_t *tcnt)
+                              const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint32_t split,
+                              uint64_t *tcnt)
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
@@ -707,7 +714,10 @@ __global__ void k_tile_counts(const uint32_t *qboff, const uint32_t *nb_dev, con
         }
         const QBlock q = qblock_of(qb, qboff, B, bstart, E);
         const int64_t jhi = last_target_block(q, elen, hilen, banded);
-        tcnt[qb] = jhi >= (int64_t)q.I ? (uint64_t)(jhi - q.I + 1) : 0u;
+        uint64_t nt = jhi >= (int64_t)q.I ? (uint64_t)(jhi - q.I + 1) : 0u;
+        // split: off-diagonal tiles of more than 32 queries become two tiles
+        if (split && nt > 1 && q.e - (q.s + q.I * IDB_TILE) > IDB_TILE / 2) nt = 2 * nt - 1;
+        tcnt[qb] = nt;
     }
 }
 
@@ -735,8 +745,8 @@ __global__ void k_size_tiles(const uint32_t *num, const uint32_t *qboff, const u
 }
 
 __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, const uint32_t *bstart, uint32_t E,
-                             const uint16_t *elen, const uint16_t *hilen, uint32_t banded, const uint64_t *ekey,
-                             uint32_t lb, uint32_t gvj, const uint64_t *toff, idb_tile *tiles)
+                             const uint16_t *elen, const uint16_t *hilen, uint32_t banded, uint32_t split,
+                             const uint64_t *ekey, uint32_t lb, uint32_t gvj, const uint64_t *toff, idb_tile *tiles)
 {
     const uint32_t B = nb_dev[0];
     const uint32_t QB = qboff[B];
@@ -755,6 +765,15 @@ __global__ void k_emit_tiles(const uint32_t *qboff, const uint32_t *nb_dev, cons
             t.q0 = q0;
             t.t0 = t0;
             t.bucket = bucket;
+            if (split && J != q.I && qn > IDB_TILE / 2) {
+                // two tiles of 32 and qn - 32 queries against the same targets
+                t.shape = (IDB_TILE / 2) | (tn << 8);
+                tiles[o++] = t;
+                t.q0 = q0 + IDB_TILE / 2;
+                t.shape = (qn - IDB_TILE / 2) | (tn << 8);
+                tiles[o++] = t;
+                continue;
+            }
             t.shape = qn | (tn << 8) | ((J == q.I ? 1u : 0u) << 16);
             tiles[o++] = t;
         }
@@ -966,6 +985,108 @@ void readback(idb_ctx *c, void *dst, const void *src, size_t bytes)
     if (staged) std::memcpy(dst, c->pinned, bytes);
 }
 
+// Host copies between caller arrays and pinned staging, split over a few
+// threads: one core's memcpy moves about 60 GB/s, and a pinned transfer runs
+// at about 45 GB/s up and down against 26 and 19 GB/s for the runtime's own
+// staging of pageable copies (MI355X host, 8 MB).  One process-wide pool;
+// calls into it are serialised.
+class CopyPool {
+public:
+    struct Job {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    static CopyPool &get()
+    {
+        static CopyPool *p = new CopyPool();  // never destroyed: its threads live until the process ends
+        return *p;
+    }
+    void run(const Job *jobs, size_t n)
+    {
+        std::lock_guard<std::mutex> call(call_m_);
+        pieces_.clear();
+        size_t total = 0;
+        for (size_t k = 0; k < n; ++k) {
+            for (size_t o = 0; o < jobs[k].bytes; o += kPiece)
+                pieces_.push_back({(uint8_t *)jobs[k].dst + o, (const uint8_t *)jobs[k].src + o,
+                                   std::min(kPiece, jobs[k].bytes - o)});
+            total += jobs[k].bytes;
+        }
+        next_.store(0);
+        if (workers_ == 0 || total < 2 * kPiece) {
+            work();
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            ++gen_;
+            active_ = workers_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return active_ == 0; });
+    }
+
+private:
+    static constexpr size_t kPiece = 256 << 10;
+    CopyPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        workers_ = std::min(3u, hw > 1 ? hw - 1 : 0u);
+        for (unsigned k = 0; k < workers_; ++k) std::thread([this] { loop(); }).detach();
+    }
+    void loop()
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> g(m_);
+            if (--active_ == 0) done_.notify_one();
+        }
+    }
+    void work()
+    {
+        for (size_t i; (i = next_.fetch_add(1)) < pieces_.size();)
+            std::memcpy(pieces_[i].dst, pieces_[i].src, pieces_[i].bytes);
+    }
+    std::mutex call_m_, m_;
+    std::condition_variable cv_, done_;
+    std::vector<Job> pieces_;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    unsigned active_ = 0, workers_ = 0;
+};
+
+// Pinned staging of at least `bytes` (at most IDB_XFER_MAX_BYTES; larger
+// transfers are copied from pageable memory directly, where the clustering
+// outweighs them).  Allocation-free calls use what an earlier call made.
+bool xfer_ready(idb_ctx *c, size_t bytes)
+{
+    if (c->xfer_cap >= bytes) return true;
+    if (c->noalloc || bytes > IDB_XFER_MAX_BYTES) return false;
+    check(c, hipStreamSynchronize(c->stream), "staging sync");
+    if (c->xfer) (void)hipHostFree(c->xfer);
+    c->xfer = nullptr;
+    c->xfer_cap = 0;
+    const size_t want = std::min<size_t>(IDB_XFER_MAX_BYTES, std::max<size_t>(bytes + bytes / 2, 1u << 20));
+    if (hipHostMalloc((void **)&c->xfer, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->xfer = nullptr;
+        return false;
+    }
+    c->xfer_cap = want;
+    return true;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
 template <typename T>
 const T *stage_in(idb_ctx *c, Slot s, const T *src, size_t count, uint32_t mem)
 {
@@ -973,6 +1094,53 @@ const T *stage_in(idb_ctx *c, Slot s, const T *src, size_t count, uint32_t mem)
     T *d = buf<T>(c, s, count);
     check(c, hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream), "input copy");
     return d;
+}
+
+// The five input arrays of a host-memory dataset in one device buffer,
+// filled by one pinned transfer (or each copied directly when the staging
+// is not available).
+struct Inputs {
+    const uint8_t *seq;
+    const uint64_t *off;
+    const uint32_t *len;
+    const int32_t *vb, *jb;
+};
+
+Inputs stage_inputs(idb_ctx *c, const idb_dataset *d, uint32_t N, uint32_t vw, uint32_t jw)
+{
+    if (d->mem == IDB_MEM_HOST) {
+        const size_t by[5] = {d->seq_bytes, (size_t)N * 8, (size_t)N * 4, (size_t)N * vw * 4, (size_t)N * jw * 4};
+        const void *src[5] = {d->seq, d->off, d->len, d->vbits, d->jbits};
+        size_t at[5], total = 0;
+        for (int k = 0; k < 5; ++k) {
+            at[k] = total;
+            total += align256(by[k]);
+        }
+        if (total && xfer_ready(c, total)) {
+            uint8_t *dev = buf<uint8_t>(c, S_SEQ, total);
+            // in four parts above 2 MiB: the transfer of one part overlaps
+            // the staging of the next
+            const size_t parts = total > (2u << 20) ? 4 : 1, step = align256((total + parts - 1) / parts);
+            for (size_t a = 0; a < total; a += step) {
+                const size_t b = std::min(total, a + step);
+                CopyPool::Job jobs[5];
+                size_t nj = 0;
+                for (int k = 0; k < 5; ++k) {
+                    const size_t lo = std::max(a, at[k]), hi = std::min(b, at[k] + by[k]);
+                    if (lo < hi) jobs[nj++] = {c->xfer + lo, (const uint8_t *)src[k] + (lo - at[k]), hi - lo};
+                }
+                CopyPool::get().run(jobs, nj);
+                check(c, hipMemcpyAsync(dev + a, c->xfer + a, b - a, hipMemcpyHostToDevice, c->stream), "input copy");
+            }
+            return {by[0] ? dev + at[0] : nullptr, (const uint64_t *)(dev + at[1]), (const uint32_t *)(dev + at[2]),
+                    by[3] ? (const int32_t *)(dev + at[3]) : nullptr, by[4] ? (const int32_t *)(dev + at[4]) : nullptr};
+        }
+    }
+    return {stage_in<uint8_t>(c, S_SEQ, d->seq, d->seq_bytes, d->mem),
+            stage_in<uint64_t>(c, S_OFF, d->off, N, d->mem),
+            stage_in<uint32_t>(c, S_LEN, d->len, N, d->mem),
+            stage_in<int32_t>(c, S_V, d->vbits, (size_t)N * vw, d->mem),
+            stage_in<int32_t>(c, S_J, d->jbits, (size_t)N * jw, d->mem)};
 }
 
 unsigned bits_for(uint64_t v)
@@ -1114,11 +1282,11 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     rng.stage("Identify Duplicates");
 
     // inputs
-    const uint8_t *seq = stage_in<uint8_t>(c, S_SEQ, d->seq, d->seq_bytes, d->mem);
-    const uint64_t *off = stage_in<uint64_t>(c, S_OFF, d->off, N, d->mem);
-    const uint32_t *len = stage_in<uint32_t>(c, S_LEN, d->len, N, d->mem);
-    const int32_t *vb = stage_in<int32_t>(c, S_V, d->vbits, (size_t)N * vw, d->mem);
-    const int32_t *jb = stage_in<int32_t>(c, S_J, d->jbits, (size_t)N * jw, d->mem);
+    const Inputs in = stage_inputs(c, d, N, vw, jw);
+    const uint8_t *seq = in.seq;
+    const uint64_t *off = in.off;
+    const uint32_t *len = in.len;
+    const int32_t *vb = in.vb, *jb = in.jb;
     if (!seq) seq = buf<uint8_t>(c, S_SEQ, 4);
     if (!vb) vb = buf<int32_t>(c, S_V, 1);
     if (!jb) jb = buf<int32_t>(c, S_J, 1);
@@ -1347,9 +1515,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         hipLaunchKernelGGL(k_bucket_blocks, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, bstart, num + 1, E, qbcnt);
         exclusive_scan_total<uint32_t>(c, qbcnt, qboff, (size_t)E + 1);
         uint64_t *tcount = buf<uint64_t>(c, S_TCNT, (size_t)E + 1);
+        const uint32_t qsplit = (p->flags & IDB_FLAG_SPLIT_TILES) ? 1u : 0u;
         uint64_t *toff = buf<uint64_t>(c, S_TOFF, (size_t)E + 1);
         hipLaunchKernelGGL(k_tile_counts, dim3(blocks_for((uint64_t)E + 1)), dim3(256), 0, s, qboff, num + 1, bstart, E,
-                           elen, hilen, banded ? 1u : 0u, tcount);
+                           elen, hilen, banded ? 1u : 0u, qsplit, tcount);
         exclusive_scan_total<uint64_t>(c, tcount, toff, (size_t)E + 1);
         hipLaunchKernelGGL(k_size_tiles, dim3(1), dim3(64), 0, s, num, qboff, toff, sz);
         uint32_t bqt[5];  // sz[2 .. 6]: buckets, query blocks, tiles (low), uniques, tiles (high)
@@ -1366,7 +1535,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         T = (uint32_t)T64;
         idb_tile *tiles = buf<idb_tile>(c, S_TILES, std::max<uint32_t>(T, 1u));
         hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(bqt[1])), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
-                           banded ? 1u : 0u, ekey2, lb, gvj, toff, tiles);
+                           banded ? 1u : 0u, qsplit, ekey2, lb, gvj, toff, tiles);
         a.tiles = tiles;
         a.n_tiles = T;
         a.ent = ent;
@@ -1434,11 +1603,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     int32_t *core = buf<int32_t>(c, S_CORE, N);
     uint32_t *lab = buf<uint32_t>(c, S_LAB, N);
     const bool dev_out = o->mem == IDB_MEM_DEVICE;
-    uint32_t *cid = dev_out ? o->clone_id : buf<uint32_t>(c, S_OUT_CID, N);
-    int32_t *core_o = dev_out ? o->core : buf<int32_t>(c, S_OUT_CORE, N);
-    uint32_t *cop_o = dev_out ? o->copies : buf<uint32_t>(c, S_OUT_COP, N);
-    uint32_t *unq_o = dev_out ? o->uniques : buf<uint32_t>(c, S_OUT_UNQ, N);
-    uint32_t *cnt_o = o->neighbour_count ? (dev_out ? o->neighbour_count : buf<uint32_t>(c, S_OUT_CNT, N)) : nullptr;
+    // host outputs: one device buffer, so that they come back in one transfer
+    const size_t ob = align256((size_t)N * 4);
+    const uint32_t n_out = o->neighbour_count ? 5u : 4u;
+    uint8_t *oall = dev_out ? nullptr : buf<uint8_t>(c, S_OUT_CID, ob * n_out);
+    uint32_t *cid = dev_out ? o->clone_id : (uint32_t *)oall;
+    int32_t *core_o = dev_out ? o->core : (int32_t *)(oall + ob);
+    uint32_t *cop_o = dev_out ? o->copies : (uint32_t *)(oall + 2 * ob);
+    uint32_t *unq_o = dev_out ? o->uniques : (uint32_t *)(oall + 3 * ob);
+    uint32_t *cnt_o = o->neighbour_count ? (dev_out ? o->neighbour_count : (uint32_t *)(oall + 4 * ob)) : nullptr;
     uint32_t *ustart = n_reps > 0 ? buf<uint32_t>(c, S_USTART, n_reps + 1) : nullptr;
     reserve_scan_tmp(c, (size_t)N + 1);
     if (E > 0 && T > 0 && (p->flags & IDB_FLAG_AUTOTUNE)) autotune(c, a, N, W, ham, learned, small, work);
@@ -1597,7 +1770,11 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     } else {
         tail(false);
     }
-    if (!dev_out) {
+    // host outputs: one pinned transfer, copied out after the final sync
+    const bool staged_out = !dev_out && xfer_ready(c, ob * n_out);
+    if (staged_out) {
+        check(c, hipMemcpyAsync(c->xfer, oall, ob * n_out, hipMemcpyDeviceToHost, s), "output copy");
+    } else if (!dev_out) {
         check(c, hipMemcpyAsync(o->clone_id, cid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
         check(c, hipMemcpyAsync(o->core, core_o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
         check(c, hipMemcpyAsync(o->copies, cop_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
@@ -1617,6 +1794,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wkc, work, sizeof(wkc), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
+    if (staged_out) {
+        CopyPool::Job jobs[5] = {{o->clone_id, c->xfer, (size_t)N * 4}, {o->core, c->xfer + ob, (size_t)N * 4},
+                                 {o->copies, c->xfer + 2 * ob, (size_t)N * 4}, {o->uniques, c->xfer + 3 * ob, (size_t)N * 4},
+                                 {o->neighbour_count, c->xfer + 4 * ob, (size_t)N * 4}};
+        CopyPool::get().run(jobs, n_out);
+    }
     for (uint32_t k = 0; k < IDB_WORK_U64 * IDB_WORK_COPIES; ++k) wk[k % IDB_WORK_U64] += wkc[k];
     if (E > 0 && (graph || !(learned || (x && x->pairs_out)))) {
         from_edges = fin[12] <= a.max_chunks;
@@ -1732,6 +1915,7 @@ void idb_destroy(idb_ctx *c)
     for (int k = 0; k < 4; ++k)
         if (c->kev[k]) (void)hipEventDestroy(c->kev[k]);
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->xfer) (void)hipHostFree(c->xfer);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

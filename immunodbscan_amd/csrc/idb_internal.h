@@ -53,6 +53,7 @@ hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1
                           uint32_t *gcnt, hipStream_t s);
 
 #define IDB_STAGE_BYTES 32768u    // each half of a context's pinned staging area
+#define IDB_XFER_MAX_BYTES (256ull << 20)  // most pinned staging a context keeps for host inputs and outputs
 // The run's small device words (zeroed at the start of a run): [12] chunks
 // handed out, [16, 67) the depth histogram, [100, 106) sizes for the host,
 // then 8 copies of the row scan's 9 words (letter set, longest row) and 16

@@ -82,6 +82,10 @@ typedef struct {
  * of 2^24 pairs or more).  Results are unchanged; it lets tests check those
  * paths on inputs that the CPU oracle finishes in seconds. */
 #define IDB_FLAG_FORCE_LARGE_PATHS 4u
+/* Off-diagonal tiles of more than 32 query entries run as two tiles of 32
+ * query rows each (the heavy tiles of dense buckets then spread over two
+ * waves); results are unchanged. */
+#define IDB_FLAG_SPLIT_TILES 8u
 
 /* Per-row outputs (cuda/Interlink.hpp:48-49 and :59-67).  uniques holds the
  * ascending representative indices in its first n_uniques entries and 0 after.

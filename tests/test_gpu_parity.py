@@ -356,6 +356,28 @@ def test_large_input_paths_forced_on_small_inputs(gpu_ctx):
             assert np.array_equal(getattr(got, k)[s:e], exp[k]), (r, k)
 
 
+def test_split_tiles_equal_the_oracle(gpu_ctx):
+    """Off-diagonal tiles of more than 32 queries run as two tiles
+    (IDB_FLAG_SPLIT_TILES): the same results on the one-word, multi-word,
+    Hamming and batched paths, with more tiles than the plain run."""
+    F = _lib.IDB_FLAG_SPLIT_TILES
+    rep = synth.generate(configs.c2_params(n_rows=20_000, n_seeds=2_000))
+    got, _ = check(rep, configs.C2, fast=True, tag="split c2", flags=F)
+    assert got.stats["tiles"] > engine.dbscan(rep, configs.C2).stats["tiles"]
+    check(synth.generate(configs.c1_params()), configs.C1, tag="split c1", flags=F)
+    rep5 = synth.generate(configs.c5_params(n_rows=20_000, n_seeds=200))
+    got, _ = check(rep5, configs.C5, fast=True, tag="split c5", flags=F)
+    assert got.stats["tiles"] > engine.dbscan(rep5, configs.C5).stats["tiles"]
+    reps = configs.c3_repertoires(n_reps=24, scale=0.05)
+    cat = synth.concat(reps)
+    got = engine.dbscan_batch(cat, configs.C3, flags=F | _lib.IDB_FLAG_GRAPH)
+    for r, rp in enumerate(reps):
+        s, e = int(cat.rep_start[r]), int(cat.rep_start[r + 1])
+        exp = oracle_of(rp, configs.C3)
+        for k in ("clone_id", "core", "copies", "uniques", "counts"):
+            assert np.array_equal(getattr(got, k)[s:e], exp[k]), (r, k)
+
+
 def test_tree_depth_histogram_measures_the_forest_before_compression(gpu_ctx):
     """The depth histogram (the reference's tree_depth_evaluation_kernel,
     PSDBSCAN.hcu:154-177, run right after cluster growth) on a forest whose
