@@ -164,13 +164,14 @@ def learned_section(args, cl, exact_rep, exact_ms, exact_res) -> dict:
                                  "flops_per_row": ENC_FLOPS_PER_ROW},
         }
 
-    if exact_rep is None:  # no headline run to compare with: time the exact path here
-        exact_rep = synth.generate(configs.c2_params(n_rows=args.rows, n_seeds=max(1, args.rows // 10)))
-        erun = engine.DeviceRun(exact_rep, cl)
-        exact_ms, _ = timed(erun, args.steps, args.warmup)
-        exact_res = erun.result()
-    out["c2"] = one(exact_rep, exact_ms, exact_res, "same as the headline line")
-    log(f"learned c2: {out['c2']['ms_per_step']:.3f} ms/step, recall {out['c2']['pair_recall']:.5f}")
+    if args.learned_c2:
+        if exact_rep is None:  # no headline run to compare with: time the exact path here
+            exact_rep = synth.generate(configs.c2_params(n_rows=args.rows, n_seeds=max(1, args.rows // 10)))
+            erun = engine.DeviceRun(exact_rep, cl)
+            exact_ms, _ = timed(erun, args.steps, args.warmup)
+            exact_res = erun.result()
+        out["c2"] = one(exact_rep, exact_ms, exact_res, "same as the headline line")
+        log(f"learned c2: {out['c2']['ms_per_step']:.3f} ms/step, recall {out['c2']['pair_recall']:.5f}")
     if args.c4_rows:
         big = synth.generate(configs.c4_params(n_rows=args.c4_rows))
         erun = engine.DeviceRun(big, cl)
@@ -208,15 +209,13 @@ def batched_section(args, threads: int) -> dict:
     log(f"batched: {len(reps)} repertoires, {cat.n} rows, {ms:.3f} ms/step")
     # CPU: the literal oracle over the first repertoires, in order, until
     # about 10 s of work; pairs/s extrapolates to the batch
-    from oracle import oracle as O  # noqa: F401  (checker only)
-
     done, secs, cpairs = 0, 0.0, 0.0
-    while done < len(reps) and secs < 10.0:
+    while not args.no_cpu_baseline and done < len(reps) and secs < 10.0:
         dt, p = cpu_literal(reps[done:done + 8], configs.C3, threads)
         secs += dt
         cpairs += p
         done += min(8, len(reps) - done)
-    cpu_v = cpairs / secs
+    cpu_v = cpairs / secs if secs else None
     return {
         "workload": f"{len(reps)} nt repertoires, lognormal sizes (median 5k, 200-200k), Hamming eps 0.15, minPts 2",
         "repertoires": len(reps),
@@ -232,10 +231,11 @@ def batched_section(args, threads: int) -> dict:
         "tuned_launch": {"waves_per_block": s["tuned_wpb"], "grid_x_resident": s["tuned_grid_mul"],
                          "xcd_order": s["tuned_xcd_order"] & 1, "candidates_timed": s["tuned_trials"]},
         "stage_ms": s["stage_ms"],
-        "cpu_baseline": {"value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
-                         "sample": f"literal all-pairs clustering of the first {done} repertoires "
-                                   f"({sum(r.n for r in reps[:done])} rows) in {secs:.1f} s; extrapolated"},
-        "gpu_over_cpu": (pairs / (ms * 1e-3)) / cpu_v,
+        "cpu_baseline": None if cpu_v is None else {
+            "value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"literal all-pairs clustering of the first {done} repertoires "
+                      f"({sum(r.n for r in reps[:done])} rows) in {secs:.1f} s; extrapolated"},
+        "gpu_over_cpu": None if cpu_v is None else (pairs / (ms * 1e-3)) / cpu_v,
     }
 
 
@@ -335,8 +335,10 @@ def large_section(args, threads: int) -> dict:
     log(f"large: {ms:.1f} ms/step")
     # CPU: the literal oracle on the first rows of the same workload
     sub_n = min(n, args.c5_cpu_rows)
-    dt, cpairs = cpu_literal([rep.subset(np.arange(sub_n))], configs.C5, threads)
-    cpu_v = cpairs / dt
+    cpu_v = None
+    if not args.no_cpu_baseline:
+        dt, cpairs = cpu_literal([rep.subset(np.arange(sub_n))], configs.C5, threads)
+        cpu_v = cpairs / dt
     return {
         "workload": f"{n} nt CDR3s, 24-150 nt, t=3 edits + V/J gate, minPts=3",
         "rows": n,
@@ -353,10 +355,10 @@ def large_section(args, threads: int) -> dict:
         "count_kernel_roofline": {"bound": "valu", "achieved": ops / (kms * 1e-3) / 1e12, "peak": VALU_PEAK_TOPS,
                                   "unit": "TOP/s", "frac": ops / (kms * 1e-3) / 1e12 / VALU_PEAK_TOPS,
                                   "kernel_ms": kms, "algorithmic_ops": ops, "filter_ops": filter_ops(s)},
-        "cpu_baseline": {"value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
-                         "sample": f"literal all-pairs clustering of the first {sub_n} rows in {dt:.1f} s; "
-                                   "extrapolated"},
-        "gpu_over_cpu": (n * (n - 1) / 2 / (ms * 1e-3)) / cpu_v,
+        "cpu_baseline": None if cpu_v is None else {
+            "value": cpu_v, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"literal all-pairs clustering of the first {sub_n} rows in {dt:.1f} s; extrapolated"},
+        "gpu_over_cpu": None if cpu_v is None else (n * (n - 1) / 2 / (ms * 1e-3)) / cpu_v,
         **({"learned": large_learned(args, rep, ms, run.result())} if args.c5_learned else {}),
     }
 
@@ -511,7 +513,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=200_000)
     ap.add_argument("--cpu-rows", type=int, default=200_000)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may use")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip every CPU timing (headline and sections); for profiled runs")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
@@ -520,6 +523,8 @@ def main() -> int:
     ap.add_argument("--train-pairs", type=int, default=5_000_000)
     ap.add_argument("--train-epochs", type=int, default=4)
     ap.add_argument("--c4-rows", type=int, default=2_000_000, help="0 skips the 2M-row learned comparison")
+    ap.add_argument("--no-learned-c2", dest="learned_c2", action="store_false",
+                    help="skip the learned filter on the 200k-row workload (the 2M-row line still runs)")
     ap.add_argument("--c3-reps", type=int, default=512)
     ap.add_argument("--c5-rows", type=int, default=20_000_000, help="0 skips the 20M-row nucleotide line")
     ap.add_argument("--c5-cpu-rows", type=int, default=40_000)

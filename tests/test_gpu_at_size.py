@@ -159,7 +159,7 @@ def test_learned_filter_at_2m_rows_recall_and_ari(gpu_ctx):
     st = enc.train(train, pairs, labels, learned.TrainParams(epochs=4, batch_pairs=4096, lr=5e-4, margin=0.9))
     print("train", st, flush=True)
     tune = synth.generate(configs.c2_params(n_rows=400_000, n_seeds=40_000, seed=43))
-    tau = learned.tune_threshold(enc.embed(tune), learned.neighbour_pairs(tune, cl), recall=0.9995, slack=0.005)
+    tau = learned.tune_threshold(enc.embed(tune), learned.neighbour_pairs(tune, cl), recall=0.9998, slack=0.01)
     test = synth.generate(configs.c4_params())
     ex = engine.dbscan(test, cl)
     le = learned.dbscan_learned(test, cl, enc, tau)
