@@ -1462,15 +1462,21 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
                            single, gj, gvj, lb, ekey, erow);
         const uint64_t max_bucket = single ? std::max(n_reps, 1u) : gvj64 * std::max(n_reps, 1u);
         const unsigned ebits = lb + bits_for(max_bucket);
-        bool sorted = false;
+        bool sorted = false, in_first = false;
         if (!force_large && E <= IDB_SORT_SMALL_MAX && ebits <= 64) {
             uint32_t *gc = buf<uint32_t>(c, S_RSCNT, (size_t)256 * idb_sort_small_blocks(E));
-            const hipError_t e = idb_sort_small(ekey, ekey2, erow, erow2, E, ebits, gc, s);
+            const hipError_t e = idb_sort_small(ekey, ekey2, erow, erow2, E, ebits, gc, s, &in_first);
             if (e == hipSuccess) sorted = true;
             else if (e == hipErrorInvalidValue) (void)hipGetLastError();
             else check(c, e, "entry sort");
         }
-        if (!sorted) sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, ebits);
+        if (!sorted) {
+            sort_pairs_u64(c, ekey, ekey2, erow, erow2, E, ebits);
+            in_first = false;
+        }
+        // the sorted entry keys and rows
+        const uint64_t *skey = in_first ? ekey : ekey2;
+        const uint32_t *srow = in_first ? erow : erow2;
         uint16_t *elen = buf<uint16_t>(c, S_ELEN, E);
         idb_entry *ent = buf<idb_entry>(c, S_ENT, E);
         uint8_t *bflag = buf<uint8_t>(c, S_BFLAG, E);
@@ -1493,7 +1499,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // entries' own genes below the bucket's (V words, then J words), when
         // they fit four words.
         uint4 *elow = (!ham && !single && !learned && vw + jw <= 4) ? buf<uint4>(c, S_ELOW, E) : nullptr;
-        hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, ekey2, erow2, E, seq, off, copies, vb, vw,
+        hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, skey, srow, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
                            ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e, elow);
         ua.pe = pe;
@@ -1535,7 +1541,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         T = (uint32_t)T64;
         idb_tile *tiles = buf<idb_tile>(c, S_TILES, std::max<uint32_t>(T, 1u));
         hipLaunchKernelGGL(k_emit_tiles, dim3(blocks_for(bqt[1])), dim3(256), 0, s, qboff, num + 1, bstart, E, elen, hilen,
-                           banded ? 1u : 0u, qsplit, ekey2, lb, gvj, toff, tiles);
+                           banded ? 1u : 0u, qsplit, skey, lb, gvj, toff, tiles);
         a.tiles = tiles;
         a.n_tiles = T;
         a.ent = ent;

@@ -44,13 +44,14 @@ struct idb_gene_hdr {
 
 // Entry order for runs of up to IDB_SORT_SMALL_MAX entries: stable radix
 // sort of (u64 key, u32 value) pairs with the keys' low end_bit bits, two
-// launches per digit pass (idb_sort.hip).  The sorted pairs end in (k1, v1);
-// gcnt holds 256 counters per block.  hipErrorInvalidValue: sizes this sort
-// does not take (nothing was launched).
+// launches per digit pass (idb_sort.hip).  The sorted pairs end in (k1, v1)
+// after an odd number of passes and back in (k0, v0) after an even one
+// (*in_first); gcnt holds 256 counters per block.  hipErrorInvalidValue:
+// sizes this sort does not take (nothing was launched).
 #define IDB_SORT_SMALL_MAX (1u << 18)
 uint32_t idb_sort_small_blocks(uint32_t n);
 hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, uint32_t n, uint32_t end_bit,
-                          uint32_t *gcnt, hipStream_t s);
+                          uint32_t *gcnt, hipStream_t s, bool *in_first);
 
 #define IDB_STAGE_BYTES 32768u    // each half of a context's pinned staging area
 #define IDB_XFER_MAX_BYTES (256ull << 20)  // most pinned staging a context keeps for host inputs and outputs

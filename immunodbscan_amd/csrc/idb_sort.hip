@@ -12,8 +12,10 @@
 //            each other with one ballot per digit bit, and a per-wave counter
 //            per digit (LDS) carries the count across the wave's slots.
 // A pair's rank counts only pairs before it in input order (lower block,
-// lower wave, lower slot, lower lane), so equal keys keep their order.  The
-// number of passes is odd, so the sorted pairs end in the second buffers.
+// lower wave, lower slot, lower lane), so equal keys keep their order.
+// Passes alternate between the two buffer pairs, as few as the key's bits
+// allow (8 bits or fewer per pass); the caller learns which pair holds the
+// result.
 #include "idb_internal.h"
 
 namespace {
@@ -142,12 +144,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *ks, c
 uint32_t idb_sort_small_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
 
 hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, uint32_t n, uint32_t end_bit,
-                          uint32_t *gcnt, hipStream_t s)
+                          uint32_t *gcnt, hipStream_t s, bool *in_first)
 {
+    *in_first = false;
     if (n == 0) return hipSuccess;
     if (n > IDB_SORT_SMALL_MAX || end_bit == 0 || end_bit > 64) return hipErrorInvalidValue;
-    uint32_t passes = (end_bit + 7) / 8;
-    passes += (passes % 2 == 0) ? 1u : 0u;
+    const uint32_t passes = (end_bit + 7) / 8;
+    *in_first = passes % 2 == 0;
     const uint32_t wd = (end_bit + passes - 1) / passes;
     if ((passes - 1) * wd >= end_bit) return hipErrorInvalidValue;  // every pass must hold a digit bit
     const uint32_t nb = idb_sort_small_blocks(n);
