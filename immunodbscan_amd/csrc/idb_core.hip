@@ -82,6 +82,10 @@ enum Slot {
 struct idb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Outputs that are final early (copies, uniques, core flags) go back to
+    // the host on this stream while the clustering runs on `stream`.
+    hipStream_t side = nullptr;
+    hipEvent_t side_ev[2] = {};
     std::string err;
     void *ptr[S_COUNT] = {};
     size_t cap[S_COUNT] = {};
@@ -887,10 +891,10 @@ __global__ void k_labels(const uint32_t *rep, const uint32_t *lab, const uint32_
         }
         noise += c == 0;
         cid[i] = c;
-        core_out[i] = core[i];
-        cop_out[i] = copies[i];
+        if (core_out) core_out[i] = core[i];  // (outputs already sent back early are skipped)
+        if (cop_out) cop_out[i] = copies[i];
         if (cnt_out) cnt_out[i] = (uint32_t)cnt[i];
-        unq_out[i] = 0u;  // the uniques' positions are written next
+        if (unq_out) unq_out[i] = 0u;  // the uniques' positions are written next
     }
     block_add(noise, small + IDB_SMALL_NOISE + (blockIdx.x & 15u));  // 16 counters
 }
@@ -1619,6 +1623,25 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     uint32_t *unq_o = dev_out ? o->uniques : (uint32_t *)(oall + 3 * ob);
     uint32_t *cnt_o = o->neighbour_count ? (dev_out ? o->neighbour_count : (uint32_t *)(oall + 4 * ob)) : nullptr;
     uint32_t *ustart = n_reps > 0 ? buf<uint32_t>(c, S_USTART, n_reps + 1) : nullptr;
+    // Host outputs: copies and uniques are final already, and the core flags
+    // are after the core-point stage (outside graph mode); they go back on
+    // the side stream into their places in the pinned staging while the
+    // clustering runs, so that only clone ids (and counts) are left for the
+    // end.
+    const bool early_out = !dev_out && c->side && xfer_ready(c, ob * n_out);
+    if (early_out) {
+        check(c, hipMemsetAsync(unq_o, 0, sizeof(uint32_t) * N, s), "memset");
+        if (n_reps > 0)
+            hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart,
+                               n_reps, ustart);
+        hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart, unq_o);
+        check(c, hipEventRecord(c->side_ev[0], s), "event record");
+        check(c, hipStreamWaitEvent(c->side, c->side_ev[0], 0), "side wait");
+        check(c, hipMemcpyAsync(c->xfer + 2 * ob, copies, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c->side),
+              "output copy");
+        check(c, hipMemcpyAsync(c->xfer + 3 * ob, unq_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c->side),
+              "output copy");
+    }
     reserve_scan_tmp(c, (size_t)N + 1);
     if (E > 0 && T > 0 && (p->flags & IDB_FLAG_AUTOTUNE)) autotune(c, a, N, W, ham, learned, small, work);
     // Learned runs stay uncaptured: when their pair list overflows, the host
@@ -1638,6 +1661,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         }
         hipLaunchKernelGGL(k_core_flags, dim3(blocks_for(N)), dim3(256), 0, s, cnt, N, (int32_t)p->min_pts, core, pe,
                            small);
+        if (!g && early_out) {
+            check(c, hipEventRecord(c->side_ev[1], s), "event record");
+            check(c, hipStreamWaitEvent(c->side, c->side_ev[1], 0), "side wait");
+            check(c, hipMemcpyAsync(c->xfer + ob, core, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->side),
+                  "output copy");
+        }
         if (!g) record(c, 3);
         rng.stage("Grow clusters");
 
@@ -1732,11 +1761,15 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         // --- Re-number clusters and identify noise
         exclusive_scan_total<uint32_t>(c, rflag, rank, (size_t)N + 1);
         hipLaunchKernelGGL(k_labels, dim3(blocks_for(N)), dim3(256), 0, s, rep, lab, rank, rid, rstart, core, copies, cnt,
-                           N, cid, core_o, cop_o, cnt_o, unq_o, small);
-        if (n_reps > 0)
-            hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart,
-                               n_reps, ustart);
-        hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart, unq_o);
+                           N, cid, (early_out && !g) ? nullptr : core_o, early_out ? nullptr : cop_o, cnt_o,
+                           early_out ? nullptr : unq_o, small);
+        if (!early_out) {
+            if (n_reps > 0)
+                hipLaunchKernelGGL(k_ustart, dim3(blocks_for((uint64_t)n_reps + 1)), dim3(256), 0, s, uniq, num, rstart,
+                                   n_reps, ustart);
+            hipLaunchKernelGGL(k_uniques_out, dim3(blocks_for(U)), dim3(256), 0, s, uniq, num, rid, rstart, ustart,
+                               unq_o);
+        }
         check(c, hipGetLastError(), "tail launch");
         if (!g) record(c, 6);
     };
@@ -1777,8 +1810,16 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         tail(false);
     }
     // host outputs: one pinned transfer, copied out after the final sync
-    const bool staged_out = !dev_out && xfer_ready(c, ob * n_out);
-    if (staged_out) {
+    const bool staged_out = early_out || (!dev_out && xfer_ready(c, ob * n_out));
+    if (early_out) {
+        // clone ids, the core flags in graph mode, and the counts
+        check(c, hipMemcpyAsync(c->xfer, cid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        if (graph)
+            check(c, hipMemcpyAsync(c->xfer + ob, core_o, sizeof(int32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
+        if (cnt_o)
+            check(c, hipMemcpyAsync(c->xfer + 4 * ob, cnt_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s),
+                  "output copy");
+    } else if (staged_out) {
         check(c, hipMemcpyAsync(c->xfer, oall, ob * n_out, hipMemcpyDeviceToHost, s), "output copy");
     } else if (!dev_out) {
         check(c, hipMemcpyAsync(o->clone_id, cid, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s), "output copy");
@@ -1800,6 +1841,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wkc, work, sizeof(wkc), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
+    if (early_out) check(c, hipStreamSynchronize(c->side), "final sync");
     if (staged_out) {
         CopyPool::Job jobs[5] = {{o->clone_id, c->xfer, (size_t)N * 4}, {o->core, c->xfer + ob, (size_t)N * 4},
                                  {o->copies, c->xfer + 2 * ob, (size_t)N * 4}, {o->uniques, c->xfer + 3 * ob, (size_t)N * 4},
@@ -1884,6 +1926,9 @@ int idb_create(idb_ctx **out, int device)
     idb_ctx *c = new idb_ctx();
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_ev[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->pinned, 2 * IDB_STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
@@ -1924,6 +1969,12 @@ void idb_destroy(idb_ctx *c)
     if (c->xfer) (void)hipHostFree(c->xfer);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    for (int k = 0; k < 2; ++k)
+        if (c->side_ev[k]) (void)hipEventDestroy(c->side_ev[k]);
     delete c;
 }
 
