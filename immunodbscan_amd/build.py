@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["idb_core.hip", "idb_pairs.hip", "idb_learned.hip", "idb_genes.hip", "idb_sort.hip"]
+HIP_SOURCES = ["idb_core.hip", "idb_pairs.hip", "idb_learned.hip", "idb_genes.hip", "idb_sort.hip", "idb_scan.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -76,7 +76,7 @@ enum Slot {
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
     S_DIST, S_CNT2, S_EQG, S_RSCNT, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
-    S_ELOW, S_COUNT
+    S_ELOW, S_SCAN, S_COUNT
 };
 
 struct idb_ctx {
@@ -101,6 +101,8 @@ struct idb_ctx {
     };
     std::map<std::array<uint32_t, 6>, Tuned> tuned;
     hipGraphExec_t gexec = nullptr;  // the tail graph, updated in place
+    void *scan_ws = nullptr;         // the current call's scan workspace (idb_scan.hip)
+    uint32_t scan_gen = 0;           // scans launched on it since it was zeroed
     // Allocation-free runs (idb_dbscan_noalloc): every buffer of the call is
     // carved from this device's scratch array instead of the arena.
     uint8_t *scratch = nullptr;   // device address of the scratch array on this device
@@ -952,31 +954,33 @@ void sort_pairs_u64(idb_ctx *c, uint64_t *kin, uint64_t *kout, uint32_t *vin, ui
 template <typename T>
 void exclusive_scan_total(idb_ctx *c, const T *in, T *out, size_t n_plus_1)
 {
-    size_t bytes = 0;
-    check(c, rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), n_plus_1, rocprim::plus<T>(), c->stream), "scan size");
-    void *t = tmp(c, bytes);
-    check(c, rocprim::exclusive_scan(t, bytes, in, out, T(0), n_plus_1, rocprim::plus<T>(), c->stream), "scan");
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if (sizeof(T) == 4)
+        check(c, idb_scan_exclusive_u32((const uint32_t *)in, (uint32_t *)out, n_plus_1, c->scan_ws, ++c->scan_gen,
+                                        c->stream), "scan");
+    else
+        check(c, idb_scan_exclusive_u64((const uint64_t *)in, (uint64_t *)out, n_plus_1, c->scan_ws, ++c->scan_gen,
+                                        c->stream), "scan");
 }
 
-// Size the arena's temporary slot for the renumbering scan ahead of a graph
-// capture (no allocation may happen while capturing).
-void reserve_scan_tmp(idb_ctx *c, size_t n_plus_1)
-{
-    size_t bytes = 0;
-    const uint32_t *in = nullptr;
-    uint32_t *out = nullptr;
-    check(c, rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, n_plus_1, rocprim::plus<uint32_t>(), c->stream),
-          "scan size");
-    (void)tmp(c, bytes);
-}
-
+// Indices of the set flags, in order, and their number (*count, device).
 void select_flagged(idb_ctx *c, size_t n, const uint8_t *flags, uint32_t *out, uint32_t *count)
 {
-    rocprim::counting_iterator<uint32_t> it(0u);
-    size_t bytes = 0;
-    check(c, rocprim::select(nullptr, bytes, it, flags, out, count, n, c->stream), "select size");
-    void *t = tmp(c, bytes);
-    check(c, rocprim::select(t, bytes, it, flags, out, count, n, c->stream), "select");
+    check(c, idb_select_flagged(flags, out, count, n, c->scan_ws, ++c->scan_gen, c->stream), "select");
+}
+
+// The scan workspace of a call, sized for scans of up to max_n values and
+// cleared when it is new (allocation-free calls carve it afresh each call).
+void *scan_workspace(idb_ctx *c, uint64_t max_n)
+{
+    const size_t bytes = idb_scan_workspace_bytes(max_n);
+    const bool fresh = c->noalloc || c->cap[S_SCAN] < bytes;
+    uint8_t *w = buf<uint8_t>(c, S_SCAN, bytes);
+    if (fresh || c->scan_gen > 0x3FFFFFFFu) {
+        check(c, hipMemsetAsync(w, 0, bytes, c->stream), "memset");
+        c->scan_gen = 0;
+    }
+    return w;
 }
 
 // Small read-backs go through the context's pinned staging area (a direct
@@ -1283,6 +1287,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     hipStream_t s = c->stream;
     record(c, 0);
     StageRanges rng;
+    // scans run over at most N + 1 rows or E + 1 entries, E <= 4N + 4096
+    c->scan_ws = scan_workspace(c, 4ull * N + 4098);
     rng.stage("Identify Duplicates");
 
     // inputs
@@ -1642,7 +1648,6 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         check(c, hipMemcpyAsync(c->xfer + 3 * ob, unq_o, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c->side),
               "output copy");
     }
-    reserve_scan_tmp(c, (size_t)N + 1);
     if (E > 0 && T > 0 && (p->flags & IDB_FLAG_AUTOTUNE)) autotune(c, a, N, W, ham, learned, small, work);
     // Learned runs stay uncaptured: when their pair list overflows, the host
     // grows it and runs the learned pass again (below).
@@ -1840,8 +1845,12 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     unsigned long long wkc[IDB_WORK_U64 * IDB_WORK_COPIES], wk[IDB_WORK_U64] = {};
     check(c, hipMemcpyAsync(fin, small, sizeof(fin), hipMemcpyDeviceToHost, s), "stats");
     check(c, hipMemcpyAsync(wkc, work, sizeof(wkc), hipMemcpyDeviceToHost, s), "stats");
+    uint32_t scan_err = 0;
+    check(c, hipMemcpyAsync(&scan_err, (uint8_t *)c->scan_ws + idb_scan_error_offset(), sizeof(scan_err),
+                            hipMemcpyDeviceToHost, s), "stats");
     check(c, hipStreamSynchronize(s), "final sync");
     if (early_out) check(c, hipStreamSynchronize(c->side), "final sync");
+    if (scan_err) fail(c, IDB_ERR_HIP, "a scan's look-back did not end (the run's sizes are not valid)");
     if (staged_out) {
         CopyPool::Job jobs[5] = {{o->clone_id, c->xfer, (size_t)N * 4}, {o->core, c->xfer + ob, (size_t)N * 4},
                                  {o->copies, c->xfer + 2 * ob, (size_t)N * 4}, {o->uniques, c->xfer + 3 * ob, (size_t)N * 4},

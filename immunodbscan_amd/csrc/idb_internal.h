@@ -49,6 +49,18 @@ struct idb_gene_hdr {
 // (*in_first); gcnt holds 256 counters per block.  hipErrorInvalidValue:
 // sizes this sort does not take (nothing was launched).
 #define IDB_SORT_SMALL_MAX (1u << 18)
+// Single-launch exclusive scans and flag selection (idb_scan.hip).  The
+// workspace (idb_scan_workspace_bytes for the largest n, zeroed when first
+// allocated) holds a ticket counter and per-tile status words; launches on
+// one workspace must be ordered (one stream), each with a generation the
+// workspace has not seen since it was zeroed (1, 2, ...).  A look-back that
+// did not end sets the 32-bit error word at idb_scan_error_offset().
+size_t idb_scan_workspace_bytes(uint64_t max_n);
+hipError_t idb_scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n, void *ws, uint32_t gen, hipStream_t s);
+hipError_t idb_scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *ws, uint32_t gen, hipStream_t s);
+hipError_t idb_select_flagged(const uint8_t *flags, uint32_t *out, uint32_t *count, uint64_t n, void *ws, uint32_t gen,
+                              hipStream_t s);
+uint32_t idb_scan_error_offset();
 uint32_t idb_sort_small_blocks(uint32_t n);
 hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, uint32_t n, uint32_t end_bit,
                           uint32_t *gcnt, hipStream_t s, bool *in_first);

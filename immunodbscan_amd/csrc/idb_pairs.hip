@@ -28,6 +28,10 @@
 // non-core row records its minimum-index core neighbour with atomicMin (the
 // canonical form of the reference's first-come atomicCAS at :113-122).
 #include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "idb_internal.h"
@@ -2057,16 +2061,34 @@ __global__ void k_union_to_rows(const uint32_t *n_chunks, uint32_t max_chunks, c
 }
 
 // Blocks that fit on the device at once for this kernel and shape.
+// Blocks of `kernel` resident on the whole device, remembered per (device,
+// kernel, block size, LDS bytes): the occupancy query is a runtime call on
+// every launch otherwise.
 uint32_t resident_blocks(const void *kernel, uint32_t block, size_t shm)
 {
-    int per_cu = 0, dev = 0, cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        dev = 0;
+    }
+    static std::mutex m;
+    static std::map<std::tuple<int, const void *, uint32_t, size_t>, uint32_t> known;
+    const auto key = std::make_tuple(dev, kernel, block, shm);
+    {
+        std::lock_guard<std::mutex> g(m);
+        const auto it = known.find(key);
+        if (it != known.end()) return it->second;
+    }
+    int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, shm) != hipSuccess || per_cu <= 0)
         per_cu = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
     (void)hipGetLastError();
-    return (uint32_t)per_cu * (uint32_t)cus;
+    const uint32_t r = (uint32_t)per_cu * (uint32_t)cus;
+    std::lock_guard<std::mutex> g(m);
+    known[key] = r;
+    return r;
 }
 
 uint32_t grid_for(uint32_t n_tiles, uint32_t wpb, uint32_t resident, uint32_t mul = 1)
@@ -2287,15 +2309,23 @@ hipError_t idb_launch_pair_distances(const uint8_t *seq, const uint64_t *off, co
 
 size_t idb_lds_limit()
 {
+    // per device, asked once
+    static std::atomic<int> known[64];
     int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0) {
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        dev = 0;
+    }
+    if (dev >= 0 && dev < 64 && known[dev].load(std::memory_order_relaxed) > 0)
+        return (size_t)known[dev].load(std::memory_order_relaxed);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0) {
         (void)hipGetLastError();
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
             (void)hipGetLastError();
             v = 65536;
         }
     }
+    if (dev >= 0 && dev < 64) known[dev].store(v, std::memory_order_relaxed);
     return (size_t)v;
 }
 
