@@ -60,9 +60,9 @@ def myers_ops_per_letter(W: int) -> int:
     return 20 * W + 4 * (W - 1)
 
 
-# absent-letter filter per pair: per direction two AND-NOTs of the presence
-# planes and two bit counts, then a maximum and the threshold compare
-FILTER_OPS_PER_PAIR = 10
+# absent-letter filter per pair: per direction an AND-NOT of the letter-set
+# planes and a bit count, then a maximum and the threshold compare
+FILTER_OPS_PER_PAIR = 6
 # 3-gram profile bound per pair: 16 four-byte absolute-difference sums, the
 # scaled threshold and the compare
 QGRAM_OPS_PER_PAIR = 18

@@ -105,7 +105,7 @@ struct __align__(16) idb_entry {
                                   // fuse pass (int16, high half)
     uint32_t len;                 // the length alone (the second half is the count pass's
                                   // query record as it stands)
-    uint32_t pres1, pres2;        // letters present once / twice (absent-letter filter)
+    uint32_t pres1, pres2;        // letters present at least once (the absent-letter filter) / twice
     uint32_t sig;                 // residues mod 16 of the genes the canonical test looks at:
                                   // V in the low half, J in the high half
 };
