@@ -280,12 +280,12 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < P; s += gridDim.x * blockDim.x)
         tab[s] = make_uint2(IDB_NONE, 0u);
     __shared__ uint32_t sym[8];
-    __shared__ uint32_t smax;
+    __shared__ uint32_t smax, s_vtop, s_jtop;
     if (threadIdx.x < 8) sym[threadIdx.x] = 0;
-    if (threadIdx.x == 0) smax = 0;
+    if (threadIdx.x == 0) smax = s_vtop = s_jtop = 0;
     __syncthreads();
     uint64_t m_lo = 0, m_hi = 0;
-    uint32_t lmax = 0;
+    uint32_t lmax = 0, vtop = 0, jtop = 0;  // longest row; highest V and J gene + 1
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t L = len[i];
         const uint8_t *s = seq + off[i];
@@ -310,8 +310,16 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
             }
         }
         h = mix64(h, acc);
-        for (uint32_t w = 0; w < vw; ++w) h = mix64(h, (uint32_t)vb[(size_t)i * vw + w]);
-        for (uint32_t w = 0; w < jw; ++w) h = mix64(h, 0x100000000ull | (uint32_t)jb[(size_t)i * jw + w]);
+        for (uint32_t w = 0; w < vw; ++w) {
+            const uint32_t b = (uint32_t)vb[(size_t)i * vw + w];
+            h = mix64(h, b);
+            if (b) vtop = max(vtop, w * 32 + 32 - (uint32_t)__builtin_clz(b));
+        }
+        for (uint32_t w = 0; w < jw; ++w) {
+            const uint32_t b = (uint32_t)jb[(size_t)i * jw + w];
+            h = mix64(h, 0x100000000ull | b);
+            if (b) jtop = max(jtop, w * 32 + 32 - (uint32_t)__builtin_clz(b));
+        }
         key[i] = fin64(h) & hmask;
         lmax = L > lmax ? L : lmax;
     }
@@ -321,6 +329,8 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
         m_hi |= __shfl_xor(m_hi, o, 64);
         const uint32_t t = __shfl_xor(lmax, o, 64);
         lmax = t > lmax ? t : lmax;
+        vtop = max(vtop, (uint32_t)__shfl_xor(vtop, o, 64));
+        jtop = max(jtop, (uint32_t)__shfl_xor(jtop, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         if (m_lo) {
@@ -332,12 +342,19 @@ __global__ void k_rowscan(const uint8_t *seq, const uint64_t *off, const uint32_
             atomicOr(&sym[3], (uint32_t)(m_hi >> 32));
         }
         atomicMax(&smax, lmax);
+        if (vtop) atomicMax(&s_vtop, vtop);
+        if (jtop) atomicMax(&s_jtop, jtop);
     }
     __syncthreads();
-    // eight copies of the nine words, so that blocks do not queue on one word
+    // eight copies of the words, so that blocks do not queue on one word
     uint32_t *dst = small + IDB_SMALL_ROWSCAN + 9 * (blockIdx.x & 7u);
     if (threadIdx.x < 8 && sym[threadIdx.x]) atomicOr(dst + threadIdx.x, sym[threadIdx.x]);
-    if (threadIdx.x == 0) atomicMax(dst + 8, smax);
+    if (threadIdx.x == 0) {
+        atomicMax(dst + 8, smax);
+        uint32_t *gd = small + IDB_SMALL_GENES + 2 * (blockIdx.x & 7u);
+        if (s_vtop) atomicMax(gd, s_vtop);
+        if (s_jtop) atomicMax(gd + 1, s_jtop);
+    }
 }
 
 __device__ bool rows_equal(const uint8_t *seq, const uint64_t *off, const uint32_t *len, const int32_t *vb, uint32_t vw,
@@ -1345,8 +1362,6 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     rng.stage("Identify core points");
 
     // --- Identify core points: buckets, tiles, count pass
-    const uint32_t gj = 32 * std::max(jw, 1u);
-    const uint64_t gvj64 = (uint64_t)32 * std::max(vw, 1u) * gj;
     // Memberships are scanned over N + 1 slots (0 past the U uniques).  The
     // entry count (sz[0..1]), U (sz[5]) and the row scan's letter set and
     // longest row (its 8 copies) come back in one read-back.
@@ -1361,7 +1376,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         exclusive_scan_total<uint64_t>(c, memb, eoff, (size_t)N + 1);
     }
     hipLaunchKernelGGL(k_size_entries, dim3(1), dim3(64), 0, s, (vw > 0 && jw > 0) ? eoff : nullptr, num, sz);
-    uint32_t hs[IDB_SMALL_ROWSCAN + 72];
+    uint32_t hs[IDB_SMALL_GENES + 16];
     readback(c, hs, small, sizeof(hs));
     uint32_t lw[9] = {};  // letter-set words and the longest row, over the row scan's 8 copies
     for (uint32_t cpy = 0; cpy < 8; ++cpy)
@@ -1370,6 +1385,17 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
             lw[t] = t < 8 ? (lw[t] | v) : std::max(lw[t], v);
         }
     const uint32_t maxlen = lw[8];
+    // Buckets are (V gene, J gene) over the gene positions that occur: bucket
+    // = v * gj + j with j < gj, so the entry key holds no more bits than the
+    // genes in use need (on 200k protein rows 9 bucket bits instead of 11, and
+    // one digit pass less in the entry sort).
+    uint32_t vtop = 0, jtop = 0;
+    for (uint32_t cpy = 0; cpy < 8; ++cpy) {
+        vtop = std::max(vtop, hs[IDB_SMALL_GENES + 2 * cpy]);
+        jtop = std::max(jtop, hs[IDB_SMALL_GENES + 2 * cpy + 1]);
+    }
+    const uint32_t gj = std::max(jtop, 1u);
+    const uint64_t gvj64 = (uint64_t)std::max(vtop, 1u) * gj;
     uint8_t cmap_h[256];
     uint32_t K = 0;
     for (uint32_t b = 0; b < 256; ++b) {

@@ -75,6 +75,7 @@ hipError_t idb_sort_small(uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1
 #define IDB_SMALL_ROWSCAN 128u
 #define IDB_SMALL_CORES 200u
 #define IDB_SMALL_NOISE 216u
+#define IDB_SMALL_GENES 232u      // 8 copies of (highest V gene + 1, highest J gene + 1), row scan
 #define IDB_SMALL_WORDS 256u
 // The run's device counters: 8 64-bit statistics
 #define IDB_WORK_U64 8u
