@@ -76,7 +76,7 @@ enum Slot {
     S_LAB, S_RANK, S_RFLAG, S_QBCNT, S_BFLAG, S_TLUT_C, S_SMALL, S_WORK, S_TMP,
     S_OUT_CID, S_OUT_CORE, S_OUT_COP, S_OUT_UNQ, S_OUT_CNT, S_USTART, S_EDGES, S_ENT, S_FILL, S_ZB, S_PAIRS,
     S_DIST, S_CNT2, S_EQG, S_RSCNT, S_PE, S_PARE, S_CMINE, S_GCALLS, S_GOFF, S_GLEN, S_GOUT, S_GHDR, S_GWORDS,
-    S_ELOW, S_SCAN, S_COUNT
+    S_ELOW, S_SCAN, S_ROWREC, S_COUNT
 };
 
 struct idb_ctx {
@@ -414,15 +414,28 @@ __global__ void k_dedup_insert(const uint8_t *seq, const uint64_t *off, const ui
 // Representatives, copies and the per-row state of the later stages: flag
 // (unique), cnt (own copies), par (own root), bmin (no core neighbour yet),
 // pe (no entry yet).
+// Also, when the row's V and J words fit in four, its 32-byte record for the
+// entry records (rrec): offset, copies, then the V words and the J words,
+// so that each entry reads one line for its row instead of four scattered
+// ones.
 __global__ void k_dedup_resolve(const uint2 *tab, const uint32_t *gslot, uint32_t n, uint32_t *rep, uint32_t *copies,
                                 uint8_t *flag, int32_t *cnt, uint32_t *par, uint32_t *bmin, uint32_t *rflag,
-                                uint32_t *pe)
+                                uint32_t *pe, const uint64_t *off, const int32_t *vb, uint32_t vw, const int32_t *jb,
+                                uint32_t jw, uint4 *rrec)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint2 g = tab[gslot[i]];
         const uint32_t cp = g.x == i ? g.y : 0u;
         rep[i] = g.x;
         copies[i] = cp;
+        if (rrec) {
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            for (uint32_t k = 0; k < vw; ++k) w[k] = (uint32_t)vb[(size_t)i * vw + k];
+            for (uint32_t k = 0; k < jw; ++k) w[vw + k] = (uint32_t)jb[(size_t)i * jw + k];
+            const uint64_t o = off[i];
+            rrec[2 * (size_t)i] = make_uint4((uint32_t)o, (uint32_t)(o >> 32), cp, 0u);
+            rrec[2 * (size_t)i + 1] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
         flag[i] = g.x == i;
         cnt[i] = (int32_t)cp;
         par[i] = i;
@@ -519,7 +532,8 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                                const uint8_t *cmap, uint32_t pad, uint32_t S, uint32_t HW, uint32_t bits,
                                const int16_t *tlut_c, const int16_t *tlut_f, uint32_t pres, uint32_t group_len, uint32_t lb,
                                uint16_t *elen, idb_entry *ent, uint32_t *ecodes, uint32_t *epk, uint8_t *bflag,
-                               uint32_t *eqg, uint32_t *pe, uint32_t *par_e, uint32_t *cmin_e, uint4 *elow)
+                               uint32_t *eqg, uint32_t *pe, uint32_t *par_e, uint32_t *cmin_e, uint4 *elow,
+                               const uint4 *rrec)
 {
     __shared__ uint8_t cm[256];
     __shared__ uint32_t qgc[256 * IDB_QGRAM_WORDS];  // one 64-byte 3-gram count row per thread
@@ -541,32 +555,82 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
         bflag[e] = (e == 0) || (group_len ? ekey[e - 1] != k : (ekey[e - 1] >> lb) != bucket);
         idb_entry r;
         r.row = u;
-        r.cp = copies[u];
+        uint64_t ro;
         uint32_t sv, sj;
-        if (single) {
-            sv = gene_residues(vb + (size_t)u * vw, vw, 0xFFFFFFFFu);
-            sj = gene_residues(jb + (size_t)u * jw, jw, 0xFFFFFFFFu);
+        const uint32_t bl = single ? 0u : (uint32_t)(bucket % gvj);
+        const uint32_t gv = single ? 0xFFFFFFFFu : bl / gj, gjb = single ? 0xFFFFFFFFu : bl % gj;
+        if (rrec) {
+            // the row's record: offset, copies, V words then J words
+            const uint4 r0 = rrec[2 * (size_t)u], r1 = rrec[2 * (size_t)u + 1];
+            ro = (uint64_t)r0.x | ((uint64_t)r0.y << 32);
+            r.cp = r0.z;
+            const uint32_t w[4] = {r1.x, r1.y, r1.z, r1.w};
+            sv = 0u;
+            sj = 0u;
+            uint32_t lw[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const bool isv = k < vw, isj = k >= vw && k < vw + jw;
+                const uint32_t kk = isv ? k : k - vw;
+                const uint32_t m = (isv || isj) ? below_gene(w[k], kk, isv ? gv : gjb) : 0u;
+                lw[k] = m;
+                const uint32_t f = (m | (m >> 16)) & 0xFFFFu;
+                if (isv) sv |= f;
+                if (isj) sj |= f;
+            }
+            if (elow) elow[e] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
         } else {
-            const uint32_t bl = (uint32_t)(bucket % gvj);
-            sv = gene_residues(vb + (size_t)u * vw, vw, bl / gj);
-            sj = gene_residues(jb + (size_t)u * jw, jw, bl % gj);
+            ro = off[u];
+            r.cp = copies[u];
+            sv = gene_residues(vb + (size_t)u * vw, vw, gv);
+            sj = gene_residues(jb + (size_t)u * jw, jw, gjb);
+            if (elow) {
+                // the row's V genes below the bucket's V gene, then its J genes
+                // below the bucket's J gene (canonical mode, vw + jw <= 4)
+                uint32_t w[4] = {0u, 0u, 0u, 0u};
+                for (uint32_t k = 0; k < vw; ++k) w[k] = below_gene((uint32_t)vb[(size_t)u * vw + k], k, gv);
+                for (uint32_t k = 0; k < jw; ++k) w[vw + k] = below_gene((uint32_t)jb[(size_t)u * jw + k], k, gjb);
+                elow[e] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
         }
         const uint32_t chk = single ? 1u : ((sv | sj) != 0u);
-        if (elow) {
-            // the row's V genes below the bucket's V gene, then its J genes
-            // below the bucket's J gene (canonical mode, vw + jw <= 4)
-            const uint32_t bl = (uint32_t)(bucket % gvj), gv = bl / gj, gjb = bl % gj;
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
-            for (uint32_t k = 0; k < vw; ++k) w[k] = below_gene((uint32_t)vb[(size_t)u * vw + k], k, gv);
-            for (uint32_t k = 0; k < jw; ++k) w[vw + k] = below_gene((uint32_t)jb[(size_t)u * jw + k], k, gjb);
-            elow[e] = make_uint4(w[0], w[1], w[2], w[3]);
-        }
         r.len_chk = L | (chk << 16);
         r.t2 = (uint32_t)(uint16_t)tlut_c[L] | ((uint32_t)(uint16_t)tlut_f[L] << 16);
         r.sig = sv | (sj << 16);
-        const uint8_t *s = seq + off[u];
+        const uint8_t *s = seq + ro;
         uint32_t p1 = 0, p2 = 0;
-        if (pres)
+        if (ecodes) {
+            // codes 16 letters at a time, each letter looked up once: letters
+            // past the row's end carry the sentinel code `pad`, and the
+            // letter-set planes come from the same codes
+            uint4 *dst = (uint4 *)(ecodes + (size_t)e * (S / 4));
+            const uint32_t padw = pad * 0x01010101u;
+            for (uint32_t k0 = 0; k0 < S; k0 += 16) {
+                if (k0 >= L) {
+                    dst[k0 / 16] = make_uint4(padw, padw, padw, padw);
+                    continue;
+                }
+                const uint4 v = row16(s, L, k0);
+                uint32_t w[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (uint32_t b = 0; b < 4; ++b) {
+                        const uint32_t pos = k0 + q * 4 + b;
+                        const uint32_t code = pos < L ? (uint32_t)cm[byte_of(v, q * 4 + b)] : pad;
+                        word |= code << (8 * b);
+                        if (pres && pos < L) {
+                            const uint32_t bit = 1u << code;
+                            p2 |= p1 & bit;
+                            p1 |= bit;
+                        }
+                    }
+                    w[q] = word;
+                }
+                dst[k0 / 16] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        } else if (pres) {
             for (uint32_t k0 = 0; k0 < L; k0 += 16) {
                 const uint4 v = row16(s, L, k0);
 #pragma unroll
@@ -577,29 +641,11 @@ __global__ void k_fill_entries(const uint64_t *ekey, const uint32_t *erow, uint3
                         p1 |= bit;
                     }
             }
+        }
         r.pres1 = p1;
         r.pres2 = p2;
         r.len = L;
         ent[e] = r;
-        if (ecodes) {
-            // letters past the row's end carry the sentinel code `pad`
-            uint4 *dst = (uint4 *)(ecodes + (size_t)e * (S / 4));
-            for (uint32_t k0 = 0; k0 < S; k0 += 16) {
-                const uint4 v = k0 < L ? row16(s, L, k0) : make_uint4(0, 0, 0, 0);
-                uint32_t w[4];
-#pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) {
-                    uint32_t word = 0;
-#pragma unroll
-                    for (uint32_t b = 0; b < 4; ++b) {
-                        const uint32_t pos = k0 + q * 4 + b;
-                        word |= (pos < L ? (uint32_t)cm[byte_of(v, q * 4 + b)] : pad) << (8 * b);
-                    }
-                    w[q] = word;
-                }
-                dst[k0 / 16] = make_uint4(w[0], w[1], w[2], w[3]);
-            }
-        }
         if (eqg) {
             // 3-gram profile (codes 0..3): counts of c[i-2] c[i-1] c[i],
             // saturating at 255
@@ -1349,8 +1395,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     uint32_t *rflag = buf<uint32_t>(c, S_RFLAG, N + 1);
     uint32_t *rank = buf<uint32_t>(c, S_RANK, N + 1);
     uint32_t *pe = buf<uint32_t>(c, S_PE, N);
+    // row records for the entry records, when a row's gene words fit in four
+    uint4 *rrec = (vw + jw <= 4) ? buf<uint4>(c, S_ROWREC, 2 * (size_t)N) : nullptr;
     hipLaunchKernelGGL(k_dedup_resolve, dim3(blocks_for(N)), dim3(256), 0, s, htab, gslot, N, rep, copies, flag, cnt, par,
-                       bmin, rflag, pe);
+                       bmin, rflag, pe, off, vb, vw, jb, jw, rrec);
     record(c, 1);
     rng.stage("Determine copies");
 
@@ -1537,7 +1585,7 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         uint4 *elow = (!ham && !single && !learned && vw + jw <= 4) ? buf<uint4>(c, S_ELOW, E) : nullptr;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, skey, srow, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
-                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e, elow);
+                           ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e, elow, rrec);
         ua.pe = pe;
         ua.par_e = par_e;
         ua.cmin_e = cmin_e;
