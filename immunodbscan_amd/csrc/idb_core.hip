@@ -1579,10 +1579,10 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
         const bool qgram = !ham && K <= 4 && maxlen >= 16;
         uint32_t *eqg = qgram ? buf<uint32_t>(c, S_EQG, (size_t)E * IDB_QGRAM_WORDS) : nullptr;
         uint32_t *par_e = buf<uint32_t>(c, S_PARE, E), *cmin_e = buf<uint32_t>(c, S_CMINE, E);
-        // The Levenshtein count passes test the canonical bucket on the
-        // entries' own genes below the bucket's (V words, then J words), when
-        // they fit four words.
-        uint4 *elow = (!ham && !single && !learned && vw + jw <= 4) ? buf<uint4>(c, S_ELOW, E) : nullptr;
+        // The count passes test the canonical bucket on the entries' own
+        // genes below the bucket's (V words, then J words), when they fit
+        // four words.
+        uint4 *elow = (!single && !learned && vw + jw <= 4) ? buf<uint4>(c, S_ELOW, E) : nullptr;
         hipLaunchKernelGGL(k_fill_entries, dim3(blocks_for(E)), dim3(256), 0, s, skey, srow, E, seq, off, copies, vb, vw,
                            jb, jw, single, gj, gvj, cmap, pad, S, HW, bits, tlut_c, tlut_f, filt ? 1u : 0u,
                            ham ? 1u : 0u, lb, elen, ent, ecodes, epk, bflag, eqg, pe, par_e, cmin_e, elow, rrec);

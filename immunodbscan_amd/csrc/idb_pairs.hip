@@ -1858,6 +1858,169 @@ __global__ __launch_bounds__(256) void k_pairs_hamming(idb_pair_args a, uint32_t
     }
 }
 
+// The count pass over Hamming groups with the exact lower-gene words
+// (canonical buckets, V and J words fitting in four).  A Hamming group is a
+// bucket's run of one length, so every pair of a tile has one length and one
+// threshold, both wave-uniform.  Queries run in windows of up to 32: each
+// lane tests its target against every query of the window with no branch,
+// the query's packed words broadcast from LDS, and a pair within the
+// threshold sets bit r - 1 - k of the lane's mask (query w0 + k, r queries
+// in the window); the exact gene test clears the bits of pairs outside their
+// canonical bucket, and the bits then become count updates and staged
+// neighbour pairs.  The gated pairs of a lane follow from the tile's shape:
+// every query (the queries below the lane on a diagonal tile), less the gene
+// test's rejects.  Replaces, for these runs, the query-at-a-time loop of
+// k_pairs_hamming (per query: lane reads of the query's fields and words, the
+// gate call and two ballots).
+template <int HWMAX, int BITS>
+__global__ __launch_bounds__(256) void k_count_ham(idb_pair_args a, uint32_t wpb)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t total_waves = gridDim.x * wpb;
+    const uint32_t wave0 = (a.no_xcd_order ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * wpb + wib;
+    const uint32_t HW = a.HW;
+    constexpr uint32_t QS = (HWMAX + 3) & ~3;  // words per query slot in LDS (whole uint4s)
+    __shared__ uint2 stage[4][IDB_TILE];
+    __shared__ int32_t qacc_s[4][IDB_TILE];
+    __shared__ uint2 qinfo_s[4][IDB_TILE];              // query row, copies
+    __shared__ uint4 qw_s[4][IDB_TILE * QS / 4];        // query words, QS per query
+    EdgeStage est{stage[wib], 0u, IDB_NONE, 0u, 0u};
+    int32_t *qacc = qacc_s[wib];
+    uint2 *qinfo = qinfo_s[wib];
+    uint4 *qw = qw_s[wib];
+    unsigned long long glane = 0, gllane = 0, hlane = 0;
+
+    for (uint32_t ti = wave0; ti < a.n_tiles; ti = next_tile(ti, total_waves)) {
+        const idb_tile tile = tile_rec(a, ti);
+        const uint32_t qn = tile.shape & 0xFFu, tn = (tile.shape >> 8) & 0xFFu;
+        const bool diag = (tile.shape >> 16) & 1u;
+        const bool tvalid = lane < tn;
+        const uint32_t p = tile.t0 + (tvalid ? lane : 0u);
+        idb_entry te{}, qe{};
+        if (tvalid) te = load_entry(a.ent + p);
+        if (lane < qn) qe = load_entry(a.ent + tile.q0 + lane);
+        uint32_t tw[HWMAX];
+#pragma unroll
+        for (int w = 0; w < HWMAX; ++w) tw[w] = 0u;
+        if (tvalid) {
+            const uint32_t *pk = a.e_pk + (size_t)p * HW;
+#pragma unroll
+            for (int w = 0; w < HWMAX; ++w)
+                if ((uint32_t)w < HW) tw[w] = pk[w];
+        }
+        {
+            // words past a row's end are zero in every packed row, so a
+            // pair's sum runs over all HWMAX words with no length test
+            uint32_t v[QS];
+#pragma unroll
+            for (int w = 0; w < (int)QS; ++w) v[w] = 0u;
+            if (lane < qn) {
+                const uint32_t *pk = a.e_pk + (size_t)(tile.q0 + lane) * HW;
+#pragma unroll
+                for (int w = 0; w < HWMAX; ++w)
+                    if ((uint32_t)w < HW) v[w] = pk[w];
+            }
+#pragma unroll
+            for (int w = 0; w < (int)QS / 4; ++w)
+                qw[lane * (QS / 4) + w] = make_uint4(v[4 * w], v[4 * w + 1], v[4 * w + 2], v[4 * w + 3]);
+        }
+        uint4 tlow = make_uint4(0u, 0u, 0u, 0u), qlow = tlow;
+        if (tvalid) tlow = a.e_low[p];
+        if (lane < qn) qlow = a.e_low[tile.q0 + lane];
+        qinfo[lane] = make_uint2(qe.row, qe.cp);
+        qacc[lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const uint32_t y = te.row, cy = te.cp;
+        const bool chk_yl = ((te.len_chk >> 16) & 1u) && (tlow.x | tlow.y | tlow.z | tlow.w) != 0u;
+        const uint64_t chkq = __ballot(lane < qn && ((qe.len_chk >> 16) & 1u));
+        // one length and one threshold per tile (lane 0 holds a target); an
+        // empty row matches every other one
+        const uint32_t L = __builtin_amdgcn_readfirstlane(te.len_chk & 0xFFFFu);
+        const uint32_t t2 = __builtin_amdgcn_readfirstlane(te.t2);
+        const int32_t tc = L ? t_of(t2, false) : 0, tf = L ? t_of(t2, true) : 0;
+        // neighbour pairs need the fuse threshold, which is below the count
+        // threshold only when it is negative (PSDBSCAN.hcu:623-636, 709-712)
+        const bool edges_ok = tf >= tc;
+        uint32_t gcnt = tvalid ? (diag ? min(lane, qn) : qn) : 0u;
+        int32_t acc_y = 0;
+        uint32_t nh = 0;
+        for (uint32_t w0 = 0; w0 < qn; w0 += 32) {
+            // r: the window's queries rounded up to whole groups of four
+            // (slots past qn fail the test below)
+            const uint32_t r = min(32u, (qn - w0 + 3u) & ~3u);
+            uint32_t mb = 0;
+            for (uint32_t g = 0; g < r; g += 4) {
+                uint4 qv[4][QS / 4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int w = 0; w < (int)QS / 4; ++w) qv[k][w] = qw[(w0 + g + k) * (QS / 4) + w];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t qi = w0 + g + k;
+                    uint32_t ham = 0;
+#pragma unroll
+                    for (int w = 0; w < HWMAX; ++w) {
+                        const uint4 q4 = qv[k][w / 4];
+                        const uint32_t xw = (w & 3) == 0 ? q4.x : ((w & 3) == 1 ? q4.y : ((w & 3) == 2 ? q4.z : q4.w));
+                        ham += BITS == 2 ? idb_mismatch2(xw, tw[w])
+                                         : (BITS == 5 ? idb_mismatch5(xw, tw[w]) : idb_mismatch8(xw, tw[w]));
+                    }
+                    const bool pass = tvalid && qi < qn && (int32_t)ham <= tc && (!diag || lane > qi);
+                    mb = mb + mb + (pass ? 1u : 0u);
+                }
+            }
+            // the exact gene test for the window's queries that need it
+            for (uint64_t cm = (chkq >> w0) & (r == 32 ? 0xFFFFFFFFull : ((1ull << r) - 1ull)); cm;
+                 cm &= cm - 1ull) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(cm), qi = w0 + k;
+                const uint32_t common = (__builtin_amdgcn_readlane(qlow.x, qi) & tlow.x) |
+                                        (__builtin_amdgcn_readlane(qlow.y, qi) & tlow.y) |
+                                        (__builtin_amdgcn_readlane(qlow.z, qi) & tlow.z) |
+                                        (__builtin_amdgcn_readlane(qlow.w, qi) & tlow.w);
+                if (tvalid && (!diag || lane > qi) && chk_yl && common != 0u) {
+                    --gcnt;
+                    mb &= ~(1u << (r - 1u - k));
+                }
+            }
+            // hits: counts of both rows, the pair staged for cluster growth
+            for (;;) {
+                const uint64_t any = __ballot(mb != 0u);
+                if (!any) break;
+                const bool has = mb != 0u;
+                const uint32_t hb = 31u - (uint32_t)__builtin_clz(mb | 1u);
+                uint32_t x = 0;
+                if (has) {
+                    const uint32_t qi = w0 + r - 1u - hb;
+                    const uint2 qinf = qinfo[qi];
+                    x = qinf.x;
+                    acc_y += (int32_t)qinf.y;
+                    atomicAdd(qacc + qi, (int32_t)cy);
+                    mb &= ~(1u << hb);
+                    ++nh;
+                }
+                stage_edges(a, est, has && edges_ok, x, y, lane);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const int32_t qa = qacc[lane];
+        if (lane < qn && qa) atomicAdd(a.cnt + qe.row, qa);
+        if (acc_y) atomicAdd(a.cnt + y, acc_y);
+        glane += gcnt;
+        gllane += (unsigned long long)gcnt * L;
+        hlane += nh;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // LDS reuse by the next tile
+    }
+    finish_edges(a, est, lane);
+    const unsigned long long gated = wave_sum_u64(glane), gletters = wave_sum_u64(gllane), hits = wave_sum_u64(hlane);
+    if (lane == 0 && a.work) {
+        atomicAdd(work_copy(a) + 0, gated);
+        atomicAdd(work_copy(a) + 1, hits);
+        atomicAdd(work_copy(a) + 3, gletters);
+    }
+}
+
 // Cluster growth over the recorded pairs, in entry space.  The union-find
 // over rows touches two scattered parents per pair (rows are in input order),
 // and its speed is set by how many scattered cache lines the waves have in
@@ -2229,10 +2392,46 @@ hipError_t launch_hamming_bits(const idb_pair_args &a, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
+// The branch-free count pass (k_count_ham) for canonical-bucket runs with
+// the exact lower-gene words and rows of at most 8 packed words.
+template <int HWMAX, int BITS>
+hipError_t launch_count_ham_hw(const idb_pair_args &a_in, hipStream_t s)
+{
+    idb_pair_args a = a_in;
+    if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
+        a.tune_wpb = 1;
+        a.tune_grid_mul = 4;
+        a.no_xcd_order = 1;
+    }
+    const uint32_t wpb = pick_wpb(a, 4);
+    const void *k = (const void *)k_count_ham<HWMAX, BITS>;
+    const dim3 block(64 * wpb), grid(grid_for(a.n_tiles, wpb, resident_blocks(k, 64 * wpb, 0), a.tune_grid_mul));
+    hipLaunchKernelGGL((k_count_ham<HWMAX, BITS>), grid, block, 0, s, a, wpb);
+    return hipGetLastError();
+}
+
+template <int BITS>
+hipError_t launch_count_ham(const idb_pair_args &a, hipStream_t s)
+{
+    // the pair sum runs over all HWMAX words: the row width exactly up to 6
+    if (a.HW <= 1) return launch_count_ham_hw<1, BITS>(a, s);
+    if (a.HW <= 2) return launch_count_ham_hw<2, BITS>(a, s);
+    if (a.HW <= 3) return launch_count_ham_hw<3, BITS>(a, s);
+    if (a.HW <= 4) return launch_count_ham_hw<4, BITS>(a, s);
+    if (a.HW <= 5) return launch_count_ham_hw<5, BITS>(a, s);
+    if (a.HW <= 6) return launch_count_ham_hw<6, BITS>(a, s);
+    return launch_count_ham_hw<8, BITS>(a, s);
+}
+
 template <bool FUSE>
 hipError_t launch_hamming(const idb_pair_args &a, hipStream_t s)
 {
     if (a.n_tiles == 0) return hipSuccess;
+    if (!FUSE && a.e_low && !a.single_bucket && a.HW <= 8) {
+        if (a.bits == 2) return launch_count_ham<2>(a, s);
+        if (a.bits == 5) return launch_count_ham<5>(a, s);
+        return launch_count_ham<8>(a, s);
+    }
     if (a.bits == 2) return launch_hamming_bits<2, FUSE>(a, s);
     if (a.bits == 5) return launch_hamming_bits<5, FUSE>(a, s);
     return launch_hamming_bits<8, FUSE>(a, s);
