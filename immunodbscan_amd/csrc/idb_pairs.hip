@@ -2397,9 +2397,12 @@ hipError_t launch_hamming_bits(const idb_pair_args &a, hipStream_t s)
 template <int HWMAX, int BITS>
 hipError_t launch_count_ham_hw(const idb_pair_args &a_in, hipStream_t s)
 {
+    // Unless a shape was tuned: 4-wave blocks, 4x the resident grid, plain
+    // order (the autotuner's pick on the 512-repertoire batch: 1.31 against
+    // 2.07 ms for one-wave blocks)
     idb_pair_args a = a_in;
     if (a.tune_wpb == 0 && a.tune_grid_mul == 0) {
-        a.tune_wpb = 1;
+        a.tune_wpb = 4;
         a.tune_grid_mul = 4;
         a.no_xcd_order = 1;
     }
