@@ -33,7 +33,7 @@
 
 namespace {
 
-constexpr uint32_t SCAN_THREADS = 256, SCAN_IPT = 8, SCAN_TILE = SCAN_THREADS * SCAN_IPT;
+constexpr uint32_t SCAN_THREADS = 256, SCAN_IPT = 16, SCAN_TILE = SCAN_THREADS * SCAN_IPT;
 constexpr uint64_t FLAG_AGG = 1, FLAG_PREFIX = 2;
 constexpr uint32_t SPIN_LIMIT = 1u << 22;
 
@@ -79,15 +79,45 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const T *in, const uint8_
     const uint32_t tile = s_tile;
     const uint64_t base = (uint64_t)tile * SCAN_TILE + (uint64_t)tid * SCAN_IPT;
 
+    // a thread's SCAN_IPT consecutive values in 16-byte loads when they are
+    // all in range (the arrays are 256-byte aligned), else one by one
     T v[SCAN_IPT];
+    const bool full = base + SCAN_IPT <= n &&
+                      ((uintptr_t)(SEL ? (const void *)flags : (const void *)in) & 15u) == 0u;
+    if (full) {
+        if (SEL) {
+            const uint4 f = *(const uint4 *)(flags + base);
+            const uint32_t fw[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+            for (uint32_t i = 0; i < SCAN_IPT; ++i) v[i] = ((fw[i >> 2] >> (8 * (i & 3))) & 0xFFu) ? T(1) : T(0);
+        } else {
+            constexpr uint32_t PER4 = 16 / sizeof(T);
+            const uint4 *src = (const uint4 *)(in + base);
+#pragma unroll
+            for (uint32_t q = 0; q < SCAN_IPT / PER4; ++q) {
+                const uint4 x = src[q];
+                if (sizeof(T) == 4) {
+                    v[4 * q] = (T)x.x;
+                    v[4 * q + 1] = (T)x.y;
+                    v[4 * q + 2] = (T)x.z;
+                    v[4 * q + 3] = (T)x.w;
+                } else {
+                    v[2 * q] = (T)(((uint64_t)x.y << 32) | x.x);
+                    v[2 * q + 1] = (T)(((uint64_t)x.w << 32) | x.z);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < SCAN_IPT; ++i) {
+            const uint64_t k = base + i;
+            if (SEL) v[i] = (k < n && flags[k]) ? T(1) : T(0);
+            else v[i] = k < n ? in[k] : T(0);
+        }
+    }
     T sum = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < SCAN_IPT; ++i) {
-        const uint64_t k = base + i;
-        if (SEL) v[i] = (k < n && flags[k]) ? T(1) : T(0);
-        else v[i] = k < n ? in[k] : T(0);
-        sum += v[i];
-    }
+    for (uint32_t i = 0; i < SCAN_IPT; ++i) sum += v[i];
     // inclusive scan of the thread sums over the wave
     T incl = sum;
 #pragma unroll
@@ -152,6 +182,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const T *in, const uint8_
     }
     __syncthreads();
     T run = s_prefix + wave_pre + (incl - sum);
+    if (!SEL && full && ((uintptr_t)out & 15u) == 0u) {
+        T o[SCAN_IPT];
+#pragma unroll
+        for (uint32_t i = 0; i < SCAN_IPT; ++i) {
+            o[i] = run;
+            run += v[i];
+        }
+        uint4 *dst = (uint4 *)(out + base);
+#pragma unroll
+        for (uint32_t q = 0; q < SCAN_IPT * sizeof(T) / 16; ++q) {
+            if (sizeof(T) == 4)
+                dst[q] = make_uint4((uint32_t)o[4 * q], (uint32_t)o[4 * q + 1], (uint32_t)o[4 * q + 2],
+                                    (uint32_t)o[4 * q + 3]);
+            else
+                dst[q] = make_uint4((uint32_t)o[2 * q], (uint32_t)((uint64_t)o[2 * q] >> 32), (uint32_t)o[2 * q + 1],
+                                    (uint32_t)((uint64_t)o[2 * q + 1] >> 32));
+        }
+        return;
+    }
 #pragma unroll
     for (uint32_t i = 0; i < SCAN_IPT; ++i) {
         const uint64_t k = base + i;
