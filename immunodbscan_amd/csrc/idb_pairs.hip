@@ -344,6 +344,40 @@ __device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &es
     }
 }
 
+// Four text letters (the bytes of `word`) through Myers; letters at or past
+// `rem` (the letters left in this lane's row) leave the state as it is.  The
+// Peq reads are under no branch, so all of them are issued before the first
+// step; a block that every active lane covers takes no per-letter select.
+// Letters past a row's end carry the pad code, whose column lies in this
+// wave's own staging area: read, never used.
+template <int W>
+__device__ __forceinline__ void myers_word(idb_myers<W> &st, uint32_t word, const uint32_t *pcol, uint32_t rem,
+                                           uint32_t top_w, uint32_t top_b)
+{
+    uint32_t eq[4][W];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int w = 0; w < W; ++w) eq[b][w] = pcol[(((word >> (8 * b)) & 0xFFu) * W + w) * 64];
+    if (!__ballot(rem < 4u)) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) idb_myers_step<W>(st, eq[b], top_w, top_b);
+    } else {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            idb_myers<W> nx = st;
+            idb_myers_step<W>(nx, eq[b], top_w, top_b);
+            const bool upd = (uint32_t)b < rem;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                st.pv[w] = upd ? nx.pv[w] : st.pv[w];
+                st.mv[w] = upd ? nx.mv[w] : st.mv[w];
+            }
+            st.score = upd ? nx.score : st.score;
+        }
+    }
+}
+
 // Myers over up to 64 queued (query, target) pairs of the current tile, one
 // pair per lane.  The target's Peq column belongs to lane tl; query and
 // target fields come from the lanes that hold them.
@@ -376,19 +410,7 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     idb_myers_init<W>(st, tm);
     const uint32_t top_w = tm ? (tm - 1) >> 5 : 0xFFFFFFFFu, top_b = tm ? (tm - 1) & 31 : 0u;
     const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
-    for (uint32_t j = 0; j < lx; j += 4) {
-        const uint32_t word = xc[j >> 2];
-#pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
-            if (j + k < lx) {
-                const uint32_t c = (word >> (8 * k)) & 0xFFu;
-                uint32_t eq[W];
-#pragma unroll
-                for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + tl];
-                idb_myers_step<W>(st, eq, top_w, top_b);
-            }
-        }
-    }
+    for (uint32_t j = 0; j < lx; j += 4) myers_word<W>(st, xc[j >> 2], peq + tl, lx - j, top_w, top_b);
     const int32_t dist = tm ? st.score : (int32_t)lx;
     const bool hit = act && dist <= t;
     if (!FUSE) hits += __popcll(__ballot(hit));
@@ -451,28 +473,95 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
     // Peq address of letter c in the window: c * (W * 64) + lo * 64 + tl words
     uint32_t pbase = (uint32_t)tl;
     // the tracked row B = min(m, 32 (lo + 2)): in word b or word a, at bit
-    // (B - 1) & 31; fixed while the window stays
+    // (B - 1) & 31; fixed while the window stays.  While the band lies
+    // inside word a, word b is untouched (its rows still hold the first
+    // column's +1 deltas), only word a is updated, and the tracked row is
+    // B' = min(m, 32 (lo + 1)) at bit bit_a: with word b untouched,
+    // D[B][j] = D[B'][j] + (B - B'), so both rows give the same tracked score
+    // and the switch between them is exact.
     bool in_b = false;
-    uint32_t bit = 0;
+    uint32_t bit = 0, bit_a = 0;
     auto place = [&]() {
         const uint32_t B = min(tm, 32u * (lo + 2u));
         in_b = B > 32u * (lo + 1u);
         bit = (B - 1u) & 31u;
+        bit_a = (min(tm, 32u * (lo + 1u)) - 1u) & 31u;
     };
     place();
     // Sixteen letters of the query per step group, from two register
     // buffers: while one group runs, the next-but-one is loaded into the
     // buffer just used (S is a multiple of 16, so every load is in the row).
+    // One text letter against the two-word window (two = true) or against
+    // word a only (the band inside word a, word b untouched).
+    auto step = [&](uint32_t c, uint32_t j, bool two) {
+        const uint32_t *pe = peq + c * (W * 64u) + pbase;
+        if (!two) {
+            const uint32_t ea = pe[0];
+            const uint32_t xva = ea | ma;
+            const uint32_t xha = (((ea & pa) + pa) ^ pa) | ea;
+            const uint32_t pha = ma | ~(xha | pa);
+            const uint32_t mha = pa & xha;
+            const int32_t dsc = (int32_t)((pha >> bit_a) & 1u) - (int32_t)((mha >> bit_a) & 1u);
+            const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
+            // selects, not a branch: a branch would sink the LDS read into
+            // it and wait for it at every letter
+            const bool upd = j <= lx;
+            score += upd ? dsc : 0;
+            pa = upd ? (mha_s | ~(xva | pha_s)) : pa;
+            ma = upd ? (pha_s & xva) : ma;
+            return;
+        }
+        const uint32_t ea = pe[0], eb = pe[64];
+        // word a: boundary row above it (shift-in +1, no add carry)
+        const uint32_t xva = ea | ma;
+        const uint64_t sa64 = (uint64_t)(ea & pa) + pa;
+        const uint32_t xha = ((uint32_t)sa64 ^ pa) | ea;
+        const uint32_t pha = ma | ~(xha | pa);
+        const uint32_t mha = pa & xha;
+        // word b: carries from word a
+        const uint32_t cin = (uint32_t)(sa64 >> 32);
+        const uint32_t xvb = eb | mb;
+        const uint32_t sb = (eb & pb) + pb + cin;
+        const uint32_t xhb = (sb ^ pb) | eb;
+        const uint32_t phb = mb | ~(xhb | pb);
+        const uint32_t mhb = pb & xhb;
+        const uint32_t phs_ = in_b ? phb : pha, mhs_ = in_b ? mhb : mha;
+        const int32_t dsc = (int32_t)((phs_ >> bit) & 1u) - (int32_t)((mhs_ >> bit) & 1u);
+        const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
+        const uint32_t phb_s = (phb << 1) | (pha >> 31), mhb_s = (mhb << 1) | (mha >> 31);
+        const bool upd = j <= lx;
+        score += upd ? dsc : 0;
+        pa = upd ? (mha_s | ~(xva | pha_s)) : pa;
+        ma = upd ? (pha_s & xva) : ma;
+        pb = upd ? (mhb_s | ~(xvb | phb_s)) : pb;
+        mb = upd ? (phb_s & xvb) : mb;
+    };
+    // window of column j: the word of row j - K, kept within [0, W - 2]
+    auto want_of = [&](uint32_t j) { return j > K + 1 ? min((j - K - 1) >> 5, (uint32_t)W - 2u) : 0u; };
+    // Four letters at a time, each block of four with no branch per letter
+    // when the window stays put over it (uniform tests): one-word steps
+    // while the band stays inside word a, else two-word steps; a block over
+    // which the window moves takes the per-letter path.
     auto group = [&](const uint4 &cur, uint32_t j0) {
 #pragma unroll
         for (uint32_t q = 0; q < 4; ++q) {
             if (j0 + 4 * q >= lmax) break;  // uniform
             const uint32_t word = q == 0 ? cur.x : (q == 1 ? cur.y : (q == 2 ? cur.z : cur.w));
+            const uint32_t jf = j0 + 4 * q + 1;  // 1-based text column of the block's first letter
+            if (want_of(jf + 3u) == lo) {
+                if (jf + 3u + K <= 32u * (lo + 1u)) {
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) step((word >> (8 * k)) & 0xFFu, jf + k, false);
+                } else {
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) step((word >> (8 * k)) & 0xFFu, jf + k, true);
+                }
+                continue;
+            }
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k) {
-                const uint32_t j = j0 + 4 * q + k + 1;  // 1-based text column (past lmax: no lane updates)
-                // window of column j: the word of row j - K, kept within [0, W - 2]
-                const uint32_t want = j > K + 1 ? min((j - K - 1) >> 5, (uint32_t)W - 2u) : 0u;
+                const uint32_t j = jf + k;  // past lmax: no lane updates
+                const uint32_t want = want_of(j);
                 if (want != lo) {  // uniform: one word further down
                     pa = pb;
                     ma = mb;
@@ -482,33 +571,7 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
                     pbase = (uint32_t)tl + lo * 64u;
                     place();
                 }
-                const uint32_t c = (word >> (8 * k)) & 0xFFu;
-                const uint32_t *pe = peq + c * (W * 64u) + pbase;
-                const uint32_t ea = pe[0], eb = pe[64];
-                // word a: boundary row above it (shift-in +1, no add carry)
-                const uint32_t xva = ea | ma;
-                const uint64_t sa64 = (uint64_t)(ea & pa) + pa;
-                const uint32_t xha = ((uint32_t)sa64 ^ pa) | ea;
-                const uint32_t pha = ma | ~(xha | pa);
-                const uint32_t mha = pa & xha;
-                // word b: carries from word a
-                const uint32_t cin = (uint32_t)(sa64 >> 32);
-                const uint32_t xvb = eb | mb;
-                const uint32_t sb = (eb & pb) + pb + cin;
-                const uint32_t xhb = (sb ^ pb) | eb;
-                const uint32_t phb = mb | ~(xhb | pb);
-                const uint32_t mhb = pb & xhb;
-                const uint32_t phs_ = in_b ? phb : pha, mhs_ = in_b ? mhb : mha;
-                const int32_t dsc = (int32_t)((phs_ >> bit) & 1u) - (int32_t)((mhs_ >> bit) & 1u);
-                const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
-                const uint32_t phb_s = (phb << 1) | (pha >> 31), mhb_s = (mhb << 1) | (mha >> 31);
-                if (j <= lx) {
-                    score += dsc;
-                    pa = mha_s | ~(xva | pha_s);
-                    ma = pha_s & xva;
-                    pb = mhb_s | ~(xvb | phb_s);
-                    mb = phb_s & xvb;
-                }
+                step((word >> (8 * k)) & 0xFFu, j, true);
             }
         }
     };
@@ -1068,19 +1131,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             idb_myers<W> st;
             idb_myers_init<W>(st, m);
             const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
-            for (uint32_t j = 0; j < lx; j += 4) {
-                const uint32_t word = xc[j >> 2];
-#pragma unroll
-                for (uint32_t k = 0; k < 4; ++k) {
-                    if (j + k < lx) {
-                        const uint32_t c = (word >> (8 * k)) & 0xFFu;
-                        uint32_t eq[W];
-#pragma unroll
-                        for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + lane];
-                        idb_myers_step<W>(st, eq, top_w, top_b);
-                    }
-                }
-            }
+            for (uint32_t j = 0; j < lx; j += 4) myers_word<W>(st, xc[j >> 2], peq + lane, lx - j, top_w, top_b);
             const int32_t dist = m ? st.score : (int32_t)lx;
             const bool hit = valid && dist <= t;
             const unsigned long long hm = __ballot(hit);
