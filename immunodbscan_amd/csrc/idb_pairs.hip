@@ -344,40 +344,6 @@ __device__ __forceinline__ void record_hit(const idb_pair_args &a, EdgeStage &es
     }
 }
 
-// Four text letters (the bytes of `word`) through Myers; letters at or past
-// `rem` (the letters left in this lane's row) leave the state as it is.  The
-// Peq reads are under no branch, so all of them are issued before the first
-// step; a block that every active lane covers takes no per-letter select.
-// Letters past a row's end carry the pad code, whose column lies in this
-// wave's own staging area: read, never used.
-template <int W>
-__device__ __forceinline__ void myers_word(idb_myers<W> &st, uint32_t word, const uint32_t *pcol, uint32_t rem,
-                                           uint32_t top_w, uint32_t top_b)
-{
-    uint32_t eq[4][W];
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int w = 0; w < W; ++w) eq[b][w] = pcol[(((word >> (8 * b)) & 0xFFu) * W + w) * 64];
-    if (!__ballot(rem < 4u)) {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) idb_myers_step<W>(st, eq[b], top_w, top_b);
-    } else {
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            idb_myers<W> nx = st;
-            idb_myers_step<W>(nx, eq[b], top_w, top_b);
-            const bool upd = (uint32_t)b < rem;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                st.pv[w] = upd ? nx.pv[w] : st.pv[w];
-                st.mv[w] = upd ? nx.mv[w] : st.mv[w];
-            }
-            st.score = upd ? nx.score : st.score;
-        }
-    }
-}
-
 // Myers over up to 64 queued (query, target) pairs of the current tile, one
 // pair per lane.  The target's Peq column belongs to lane tl; query and
 // target fields come from the lanes that hold them.
@@ -410,7 +376,19 @@ __device__ __forceinline__ void myers_batch(const idb_pair_args &a, const idb_ti
     idb_myers_init<W>(st, tm);
     const uint32_t top_w = tm ? (tm - 1) >> 5 : 0xFFFFFFFFu, top_b = tm ? (tm - 1) & 31 : 0u;
     const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
-    for (uint32_t j = 0; j < lx; j += 4) myers_word<W>(st, xc[j >> 2], peq + tl, lx - j, top_w, top_b);
+    for (uint32_t j = 0; j < lx; j += 4) {
+        const uint32_t word = xc[j >> 2];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (j + k < lx) {
+                const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                uint32_t eq[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + tl];
+                idb_myers_step<W>(st, eq, top_w, top_b);
+            }
+        }
+    }
     const int32_t dist = tm ? st.score : (int32_t)lx;
     const bool hit = act && dist <= t;
     if (!FUSE) hits += __popcll(__ballot(hit));
@@ -503,8 +481,7 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
             const uint32_t mha = pa & xha;
             const int32_t dsc = (int32_t)((pha >> bit_a) & 1u) - (int32_t)((mha >> bit_a) & 1u);
             const uint32_t pha_s = (pha << 1) | 1u, mha_s = mha << 1;
-            // selects, not a branch: a branch would sink the LDS read into
-            // it and wait for it at every letter
+            // letters past the row's end: selects keep the state
             const bool upd = j <= lx;
             score += upd ? dsc : 0;
             pa = upd ? (mha_s | ~(xva | pha_s)) : pa;
@@ -1131,7 +1108,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             idb_myers<W> st;
             idb_myers_init<W>(st, m);
             const uint32_t *xc = (const uint32_t *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
-            for (uint32_t j = 0; j < lx; j += 4) myers_word<W>(st, xc[j >> 2], peq + lane, lx - j, top_w, top_b);
+            for (uint32_t j = 0; j < lx; j += 4) {
+                const uint32_t word = xc[j >> 2];
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) {
+                    if (j + k < lx) {
+                        const uint32_t c = (word >> (8 * k)) & 0xFFu;
+                        uint32_t eq[W];
+#pragma unroll
+                        for (int w = 0; w < W; ++w) eq[w] = peq[(c * W + w) * 64 + lane];
+                        idb_myers_step<W>(st, eq, top_w, top_b);
+                    }
+                }
+            }
             const int32_t dist = m ? st.score : (int32_t)lx;
             const bool hit = valid && dist <= t;
             const unsigned long long hm = __ballot(hit);
