@@ -688,8 +688,12 @@ __device__ __forceinline__ void myers_batch1(const idb_pair_args &a, const uint3
 //     counts per row (saturation only lowers the L1 distance, so the bound
 //     stays a lower bound); the L1 distance is 16 v_sad_u8 per pair, query
 //     profiles are read from LDS by broadcast.
+// Occupancy: 4 waves per SIMD (128 VGPRs) in general; the 3-gram path with
+// banded Myers (3 or more words) needs about 150 and runs at 3, which on 20M
+// nucleotide rows is 12% faster than 4 waves with registers spilled.
 template <int W, bool FUSE, int FILT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_myers(idb_pair_args a, uint32_t wpb)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FILT == 2 && W >= 3 ? 3 : 4))) void k_pairs_myers(
+    idb_pair_args a, uint32_t wpb)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = lane_id();
