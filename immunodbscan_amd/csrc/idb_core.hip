@@ -1337,6 +1337,8 @@ void run(idb_ctx *c, const idb_dataset *d, const uint32_t *rstart_h, uint32_t n_
     if (N && (!d->off || !d->len || (d->seq_bytes && !d->seq))) fail(c, IDB_ERR_ARG, "missing input array");
     if (N && ((vw && !d->vbits) || (jw && !d->jbits))) fail(c, IDB_ERR_ARG, "missing bitset array");
     if (vw > 64 || jw > 64) fail(c, IDB_ERR_UNSUPPORTED, "gene bitsets wider than 64 words (2048 genes)");
+    // the duplicate table holds at least 2N slots under 32-bit indexes
+    if (N >= (1u << 31)) fail(c, IDB_ERR_UNSUPPORTED, "2^31 rows or more in one call");
     if (p->debug_hash_bits > 64) fail(c, IDB_ERR_ARG, "debug_hash_bits must be 0..64");
     if (st) {
         std::memset(st, 0, sizeof(*st));
