@@ -44,12 +44,8 @@ constexpr uint32_t O_EMB = 0, O_W1 = O_EMB + NTOK * C0, O_B1 = O_W1 + C * T * C0
                    O_BP = O_WP + D * C, N_PARAMS = O_BP + D;
 static_assert(N_PARAMS == IDB_ENC_PARAMS, "parameter count");
 
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even (the hardware conversion, one instruction).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // tanh(u) = 1 - 2 / (exp(2u) + 1): one hardware exponential and one
@@ -58,9 +54,14 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 // rounding every activation goes through.
 __device__ __forceinline__ float tanh_exp(float u) { return 1.f - __fdividef(2.f, __expf(2.f * u) + 1.f); }
 
+// 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)), u = k (x + 0.044715 x^3): one
+// hardware base-2 exponential and one reciprocal.  For very negative x the
+// exponential overflows to inf and the result is -0; for large x it is x.
 __device__ __forceinline__ float gelu(float x)
 {
-    return 0.5f * x * (1.f + tanh_exp(GELU_K * (x + 0.044715f * x * x * x)));
+    constexpr float C1 = -2.f * GELU_K * 1.4426950408889634f, C3 = C1 * 0.044715f;  // -2u log2(e) = x (C1 + C3 x^2)
+    const float e = x * __builtin_fmaf(C3, x * x, C1);
+    return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(e));
 }
 __device__ __forceinline__ float gelu_grad(float x)
 {
@@ -259,14 +260,26 @@ struct EpConvFwd {
     {
         if (col >= N || row0 >= M) return;
         const float b = bf2f(bias[col]);
-        const uint32_t L = len[row0 / PP], p0 = row0 % PP;
+        // rows row0 + pl + d(r), d(r) = 8 (r / 4) + r % 4, within one sequence
+        const uint32_t pl = (lane >> 5) * 4;
+        const int32_t lim = (int32_t)len[row0 / PP] - (int32_t)(row0 % PP + pl);  // rows d < lim are letters
+        const size_t base = (size_t)(row0 + pl) * C + col;
+        uint16_t *a = A + base;
+        if (H) {
+            uint16_t *h = H + base;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t p = tile_row(r, lane);
-            const size_t at = (size_t)(row0 + p) * C + col;
-            const float x = v[r] + b;
-            if (H) H[at] = f2bf(x);
-            A[at] = p0 + p < L ? f2bf(gelu(x)) : (uint16_t)0;
+            for (int r = 0; r < 16; ++r) {
+                const int32_t d = (r >> 2) * 8 + (r & 3);
+                const float x = v[r] + b;
+                h[d * C] = f2bf(x);
+                a[d * C] = d < lim ? f2bf(gelu(x)) : (uint16_t)0;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int32_t d = (r >> 2) * 8 + (r & 3);
+                a[d * C] = d < lim ? f2bf(gelu(v[r] + b)) : (uint16_t)0;
+            }
         }
     }
 };
