@@ -190,6 +190,24 @@ def test_unsupported_length_is_a_clean_error(gpu_ctx):
     check(rep_from_rows(rows[:1]), configs.C1, tag="after error")
 
 
+def test_row_count_past_the_slot_indexes_is_a_clean_error(gpu_ctx):
+    """2^31 rows or more are refused before any input is read: the arrays here hold one row."""
+    ctx = _lib.Context(0)
+    seq = np.zeros(16, np.uint8)
+    off = np.zeros(1, np.uint64)
+    length = np.ones(1, np.uint32)
+    bits = np.zeros(1, np.int32)
+    d = _lib.make_dataset(1 << 31, seq, off, length, bits, 1, bits, 1)
+    prm = engine._params(configs.C2, False, 0, 0, 0)
+    outs = _lib.make_outputs(*[np.zeros(1, np.uint32) for _ in range(4)])
+    print("case n_rows=2^31 with one-row arrays", flush=True)
+    assert _lib.lib.idb_dbscan(ctx.handle, d, prm, outs, None) == _lib.IDB_ERR_UNSUPPORTED
+    assert "2^31" in ctx.last_error()
+    rows = [("CASSLG", [0], [0]), ("CASSLA", [0], [0])]
+    check(rep_from_rows(rows), configs.C2, ctx=ctx, tag="after error")
+    ctx.close()
+
+
 # --------------------------------------------------------------------------
 # Batch, boundary and device-resident paths
 def test_batch_equals_separate_runs(gpu_ctx):
