@@ -1076,28 +1076,43 @@ public:
     void run(const Job *jobs, size_t n)
     {
         std::lock_guard<std::mutex> call(call_m_);
-        pieces_.clear();
-        size_t total = 0;
-        for (size_t k = 0; k < n; ++k) {
-            for (size_t o = 0; o < jobs[k].bytes; o += kPiece)
-                pieces_.push_back({(uint8_t *)jobs[k].dst + o, (const uint8_t *)jobs[k].src + o,
-                                   std::min(kPiece, jobs[k].bytes - o)});
-            total += jobs[k].bytes;
-        }
-        next_.store(0);
-        if (workers_ == 0 || total < 2 * kPiece) {
-            work();
-            return;
-        }
-        {
-            std::lock_guard<std::mutex> g(m_);
-            ++gen_;
-            active_ = workers_;
-        }
-        cv_.notify_all();
+        split(jobs, nullptr, n, 1);
+        start();
         work();
-        std::unique_lock<std::mutex> g(m_);
-        done_.wait(g, [&] { return active_ == 0; });
+        finish();
+    }
+    // Copies like run(); job k belongs to part part_of[k] (non-decreasing,
+    // below kMaxParts).  The calling thread calls issue(p) for p = 0, 1, ...
+    // in order, each as soon as every byte of part p is copied, while the
+    // workers go on with the later parts: one wake-up of the pool per call.
+    static constexpr uint32_t kMaxParts = 8;
+    template <class F>
+    void run_parts(const Job *jobs, const uint32_t *part_of, size_t n, uint32_t parts, F &&issue)
+    {
+        std::lock_guard<std::mutex> call(call_m_);
+        split(jobs, part_of, n, parts);
+        start();
+        uint32_t issued = 0;
+        auto flush = [&] {
+            while (issued < parts && left_[issued].load(std::memory_order_acquire) == 0) issue(issued++);
+        };
+        try {
+            for (size_t i; (i = next_.fetch_add(1)) < pieces_.size();) {
+                copy(i);
+                flush();
+            }
+            while (issued < parts) {
+                flush();
+                if (issued < parts) std::this_thread::yield();
+            }
+        } catch (...) {
+            // a failed issue: the workers stop claiming pieces and are
+            // waited for before the pieces they read go away
+            next_.store(pieces_.size());
+            finish();
+            throw;
+        }
+        finish();
     }
 
 private:
@@ -1122,15 +1137,61 @@ private:
             if (--active_ == 0) done_.notify_one();
         }
     }
+    // pieces of at most kPiece bytes, in job order, with their parts
+    void split(const Job *jobs, const uint32_t *part_of, size_t n, uint32_t parts)
+    {
+        pieces_.clear();
+        piece_part_.clear();
+        total_ = 0;
+        size_t count[kMaxParts] = {};
+        for (size_t k = 0; k < n; ++k) {
+            const uint32_t part = part_of ? part_of[k] : 0u;
+            for (size_t o = 0; o < jobs[k].bytes; o += kPiece) {
+                pieces_.push_back({(uint8_t *)jobs[k].dst + o, (const uint8_t *)jobs[k].src + o,
+                                   std::min(kPiece, jobs[k].bytes - o)});
+                piece_part_.push_back(part);
+                ++count[part];
+            }
+            total_ += jobs[k].bytes;
+        }
+        for (uint32_t q = 0; q < kMaxParts; ++q) left_[q].store(q < parts ? count[q] : 0, std::memory_order_relaxed);
+        next_.store(0);
+    }
+    // wakes the workers when the copy is worth sharing
+    void start()
+    {
+        woke_ = workers_ > 0 && total_ >= 2 * kPiece;
+        if (!woke_) return;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            ++gen_;
+            active_ = workers_;
+        }
+        cv_.notify_all();
+    }
+    void finish()
+    {
+        if (!woke_) return;
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return active_ == 0; });
+    }
+    void copy(size_t i)
+    {
+        std::memcpy(pieces_[i].dst, pieces_[i].src, pieces_[i].bytes);
+        left_[piece_part_[i]].fetch_sub(1, std::memory_order_release);
+    }
     void work()
     {
-        for (size_t i; (i = next_.fetch_add(1)) < pieces_.size();)
-            std::memcpy(pieces_[i].dst, pieces_[i].src, pieces_[i].bytes);
+        for (size_t i; (i = next_.fetch_add(1)) < pieces_.size();) copy(i);
     }
     std::mutex call_m_, m_;
     std::condition_variable cv_, done_;
     std::vector<Job> pieces_;
+    std::vector<uint32_t> piece_part_;
+    std::atomic<size_t> left_[kMaxParts];  // pieces of each part not yet copied
     std::atomic<size_t> next_{0};
+    size_t total_ = 0;
+    bool woke_ = false;
     uint64_t gen_ = 0;
     unsigned active_ = 0, workers_ = 0;
 };
@@ -1190,19 +1251,27 @@ Inputs stage_inputs(idb_ctx *c, const idb_dataset *d, uint32_t N, uint32_t vw, u
         if (total && xfer_ready(c, total)) {
             uint8_t *dev = buf<uint8_t>(c, S_SEQ, total);
             // in four parts above 2 MiB: the transfer of one part overlaps
-            // the staging of the next
+            // the staging of the next, and each part goes out as soon as it
+            // is staged
             const size_t parts = total > (2u << 20) ? 4 : 1, step = align256((total + parts - 1) / parts);
-            for (size_t a = 0; a < total; a += step) {
+            CopyPool::Job jobs[4 * 5];
+            uint32_t part_of[4 * 5];
+            size_t nj = 0;
+            uint32_t np = 0;
+            for (size_t a = 0; a < total; a += step, ++np) {
                 const size_t b = std::min(total, a + step);
-                CopyPool::Job jobs[5];
-                size_t nj = 0;
                 for (int k = 0; k < 5; ++k) {
                     const size_t lo = std::max(a, at[k]), hi = std::min(b, at[k] + by[k]);
-                    if (lo < hi) jobs[nj++] = {c->xfer + lo, (const uint8_t *)src[k] + (lo - at[k]), hi - lo};
+                    if (lo < hi) {
+                        jobs[nj] = {c->xfer + lo, (const uint8_t *)src[k] + (lo - at[k]), hi - lo};
+                        part_of[nj++] = np;
+                    }
                 }
-                CopyPool::get().run(jobs, nj);
-                check(c, hipMemcpyAsync(dev + a, c->xfer + a, b - a, hipMemcpyHostToDevice, c->stream), "input copy");
             }
+            CopyPool::get().run_parts(jobs, part_of, nj, np, [&](uint32_t q) {
+                const size_t a = q * step, b = std::min(total, a + step);
+                check(c, hipMemcpyAsync(dev + a, c->xfer + a, b - a, hipMemcpyHostToDevice, c->stream), "input copy");
+            });
             return {by[0] ? dev + at[0] : nullptr, (const uint64_t *)(dev + at[1]), (const uint32_t *)(dev + at[2]),
                     by[3] ? (const int32_t *)(dev + at[3]) : nullptr, by[4] ? (const int32_t *)(dev + at[4]) : nullptr};
         }
