@@ -12,6 +12,7 @@ import subprocess
 import tempfile
 
 import numpy as np
+import pytest
 
 from immunodbscan_amd import configs, synth
 from oracle import oracle as O
@@ -124,3 +125,19 @@ def test_pair_primitives_on_host():
         r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
     assert "OK mismatches=0" in r.stdout
+
+
+@pytest.mark.parametrize("flags", [["-O2"], ["-O1", "-g", "-fsanitize=thread"]], ids=["O2", "tsan"])
+def test_copy_pool_parts(flags):
+    """The host copy pool behind the staged input transfer (one pass, parts handed over in order)."""
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "t")
+        b = subprocess.run(["g++", *flags, "-std=c++17", "-pthread", os.path.join(ROOT, "tests", "cpp", "test_copy_pool.cpp"),
+                            "-o", exe], capture_output=True, text=True)
+        if b.returncode != 0 and "-fsanitize=thread" in flags:
+            pytest.skip("no thread sanitizer in this toolchain: " + b.stderr[-200:])
+        assert b.returncode == 0, b.stderr
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK copy pool" in r.stdout
+
