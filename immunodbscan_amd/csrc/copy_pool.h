@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 #include <vector>
 
@@ -44,6 +45,10 @@ public:
     void run_parts(const Job *jobs, const uint32_t *part_of, size_t n, uint32_t parts, F &&issue)
     {
         std::lock_guard<std::mutex> call(call_m_);
+        if (parts > kMaxParts) throw std::length_error("copy pool: more parts than it tracks");
+        for (size_t k = 0; k < n; ++k)
+            if (part_of[k] >= parts || (k > 0 && part_of[k] < part_of[k - 1]))
+                throw std::invalid_argument("copy pool: job parts must be non-decreasing and below the part count");
         split(jobs, part_of, n, parts);
         start();
         uint32_t issued = 0;

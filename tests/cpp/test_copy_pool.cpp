@@ -71,6 +71,27 @@ int main()
             }
         }
     }
+    // part lists the pool cannot track are refused before any copy
+    {
+        std::vector<uint8_t> a(64), b(64);
+        CopyPool::Job j[2] = {{a.data(), b.data(), 32}, {a.data() + 32, b.data() + 32, 32}};
+        const uint32_t bad_order[2] = {1, 0}, too_many[2] = {0, 9};
+        int refused = 0;
+        try {
+            CopyPool::get().run_parts(j, bad_order, 2, 2, [](uint32_t) {});
+        } catch (const std::invalid_argument &) {
+            ++refused;
+        }
+        try {
+            CopyPool::get().run_parts(j, too_many, 2, 10, [](uint32_t) {});
+        } catch (const std::length_error &) {
+            ++refused;
+        }
+        if (refused != 2) {
+            printf("FAIL bad part lists accepted\n");
+            return 1;
+        }
+    }
     printf("OK copy pool\n");
     return 0;
 }
