@@ -466,9 +466,6 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
         bit_a = (min(tm, 32u * (lo + 1u)) - 1u) & 31u;
     };
     place();
-    // Sixteen letters of the query per step group, from two register
-    // buffers: while one group runs, the next-but-one is loaded into the
-    // buffer just used (S is a multiple of 16, so every load is in the row).
     // One text letter against the two-word window (two = true) or against
     // word a only (the band inside word a, word b untouched).
     auto step = [&](uint32_t c, uint32_t j, bool two) {
@@ -552,6 +549,9 @@ __device__ __forceinline__ void myers_batch_band(const idb_pair_args &a, const i
             }
         }
     };
+    // Sixteen letters of the query per step group, from two register
+    // buffers: while one group runs, the next-but-one is loaded into the
+    // buffer just used (S is a multiple of 16, so every load is in the row).
     const uint4 *xq = (const uint4 *)(a.e_codes + (size_t)(tile.q0 + qi) * a.S);
     const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
     uint4 bufa = lx ? xq[0] : zero4, bufb = lx > 16 ? xq[1] : zero4;
